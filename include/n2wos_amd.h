@@ -1,0 +1,143 @@
+/*
+ * n2wos_amd.h -- C ABI of the MI355X (gfx950) Walk-on-Spheres engine.
+ *
+ * Conventions
+ *   - Every entry point returns 0 (N2WOS_OK) or a negative N2WOS_ERR_* code;
+ *     n2wos_last_error() then describes the failure (thread-local string).
+ *     No C++ exception crosses this boundary.
+ *   - Pointers named d_* are HIP device pointers owned by the caller.  Calls
+ *     that take a stream only enqueue work on it: they neither allocate,
+ *     transfer nor synchronise unless their comment says otherwise.
+ *   - Points are arrays of float triples (x, y, z), 12 bytes per point, the
+ *     element layout of the reference's DeviceVec3 buffers.
+ *   - `stream` is a hipStream_t passed as void*; NULL is the null stream.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   n2wos_bvh_create / _destroy      CudaBvh::CudaBvh / ~CudaBvh
+ *                                    (src/cuda/bvh_query.cu:171-226,
+ *                                    include/n2wos/cuda_bvh.hpp:39-42) and
+ *                                    CuBqlBvh::CuBqlBvh (include/n2wos/cubql_bvh.hpp:33-35)
+ *   n2wos_bvh_query_device           CudaBvh::query_device (include/n2wos/cuda_bvh.hpp:50-57,
+ *                                    src/cuda/bvh_query.cu:228-261)
+ *   n2wos_bvh_query_device_masked    CuBqlBvh::query_device_masked
+ *                                    (include/n2wos/cubql_bvh.hpp:57-65,
+ *                                    src/cuda/cubql_bvh.cu:680-714)
+ *   n2wos_bvh_get_stats              triangle_count/node_count/index_count/leaf_size/
+ *                                    max_depth accessors (include/n2wos/cuda_bvh.hpp:59-63)
+ *   n2wos_uv_sphere                  make_procedural_bumpy_sphere
+ *                                    (src/mesh/procedural_bumpy_sphere.cpp:32-90)
+ *   n2wos_normalize_to_unit_radius   normalize_to_unit_radius (src/mesh/normalize.cpp:18-37)
+ *   n2wos_count_degenerate_triangles count_degenerate_triangles
+ *                                    (src/geometry/closest_point.cpp:70-83)
+ */
+#ifndef N2WOS_AMD_H
+#define N2WOS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define N2WOS_ABI_VERSION 1
+
+enum {
+  N2WOS_OK = 0,
+  N2WOS_ERR_INVALID = -1,     /* bad argument value (negative count, bad option) */
+  N2WOS_ERR_NULL = -2,        /* required pointer is NULL */
+  N2WOS_ERR_BLOCK = -3,       /* block size outside [1, 1024] */
+  N2WOS_ERR_HIP = -4,         /* HIP runtime or launch failure */
+  N2WOS_ERR_EMPTY = -5,       /* handle holds no geometry */
+  N2WOS_ERR_NOMEM = -6,       /* host or device allocation failed */
+  N2WOS_ERR_UNSUPPORTED = -7, /* option not implemented (e.g. build method) */
+  N2WOS_ERR_INTERNAL = -8
+};
+
+typedef struct n2wos_bvh n2wos_bvh;
+
+/* Description of the last failure on the calling thread ("" if none). */
+const char* n2wos_last_error(void);
+int n2wos_abi_version(void);
+/* Device name and runtime version of the current HIP device, for reports. */
+int n2wos_device_summary(char* buf, int buf_len);
+
+/* ----------------------------------------------------------------- meshes */
+/* UV-sphere radius kinds. */
+enum {
+  N2WOS_SPHERE_REFERENCE_BUMP = 0, /* reference bump_radius, params[0] = amplitude */
+  N2WOS_SPHERE_SIN6 = 1,           /* 1 + params[0] sin(6 theta) sin(6 phi) */
+  N2WOS_SPHERE_MULTI = 2,          /* 1 + .06 sin5t sin7p + .03 sin17t cos13p */
+  N2WOS_SPHERE_SH_BLOB = 3         /* 1 + sum_{l<=8} params[l*l+l+m] Y_lm */
+};
+/* Vertex and triangle counts: nv = 2 + (stacks-1) slices, nt = 2 slices (stacks-1). */
+int n2wos_uv_sphere_counts(int stacks, int slices, int* nv, int* nt);
+/* Fills verts[3 nv] and tris[3 nt] (vertex and face order of the reference). */
+int n2wos_uv_sphere(int stacks, int slices, int kind, const float* params, int n_params, float* verts,
+                    uint32_t* tris);
+/* In place: p <- (p - c) / R, c = AABB midpoint, R = max |p - c|.  Writes c and 1/R. */
+int n2wos_normalize_to_unit_radius(float* verts, int nv, float* center, float* scale);
+/* Number of triangles with |(b-a) x (c-a)|^2 <= area2_eps. */
+int64_t n2wos_count_degenerate_triangles(const float* verts, int nv, const uint32_t* tris, int nt,
+                                         float area2_eps);
+
+/* -------------------------------------------------------------------- BVH */
+/* Builds the BVH on the host and uploads it to the current device.
+ * build_method: NULL, "median" or "custom" (median split, leaves of at most
+ * leaf_size triangles, leaf_size in [1, 31]); others -> N2WOS_ERR_UNSUPPORTED. */
+int n2wos_bvh_create(const float* verts, int nv, const uint32_t* tris, int nt, int leaf_size,
+                     const char* build_method, n2wos_bvh** out);
+int n2wos_bvh_destroy(n2wos_bvh* bvh);
+
+typedef struct {
+  int64_t triangle_count;
+  int64_t node_count;      /* 4-wide nodes */
+  int64_t index_count;     /* triangle records (== triangle_count) */
+  int64_t binary_nodes;    /* nodes of the median-split tree before collapsing */
+  int leaf_size;
+  int max_depth;           /* depth of the median-split tree */
+  int depth4;              /* depth of the 4-wide tree */
+  float prune_margin;      /* absolute distance margin used for exact pruning */
+  double build_milliseconds;
+  int64_t device_bytes;
+  float bounds_lo[3];
+  float bounds_hi[3];
+  char build_method[32];
+} n2wos_bvh_stats;
+int n2wos_bvh_get_stats(const n2wos_bvh* bvh, n2wos_bvh_stats* out);
+
+/* Closest point on the mesh for each of query_count points.
+ * Outputs: squared distance, closest point (3 floats), original triangle
+ * index (lowest index among exact ties), overflow flag (0 unless the
+ * traversal stack overflowed and the result may be inexact).  A point with a
+ * non-finite coordinate yields d2 = FLT_MAX, closest = (0,0,0), tri = -1.
+ * query_count == 0 is a no-op; < 0 -> N2WOS_ERR_INVALID. */
+int n2wos_bvh_query_device(const n2wos_bvh* bvh, const float* d_points, int query_count, float* d_distance2,
+                           float* d_closest, int* d_triangle_id, int* d_overflow, int block_size,
+                           void* stream);
+/* As above; where d_active[i] == 0 the outputs are d2 = 0, closest = the
+ * point itself, tri = -1, overflow = 0 and no traversal runs. */
+int n2wos_bvh_query_device_masked(const n2wos_bvh* bvh, const float* d_points, const uint8_t* d_active,
+                                  int query_count, float* d_distance2, float* d_closest, int* d_triangle_id,
+                                  int* d_overflow, int block_size, void* stream);
+/* Query with per-query visit counters: d_node_visits[i] / d_tri_visits[i]
+ * receive the number of node lines loaded and triangle records tested (the
+ * inputs of the traffic model).  Same results as n2wos_bvh_query_device. */
+int n2wos_bvh_query_device_counted(const n2wos_bvh* bvh, const float* d_points, int query_count,
+                                   float* d_distance2, float* d_closest, int* d_triangle_id, int* d_overflow,
+                                   uint32_t* d_node_visits, uint32_t* d_tri_visits, int block_size,
+                                   void* stream);
+/* Host-only build (no device needed): the node and triangle-record arrays
+ * exactly as n2wos_bvh_create would upload them.  Call with nodes_out ==
+ * NULL to get *node_count; tris_out holds nt records of 48 bytes.
+ * *prune_margin receives the absolute pruning margin. */
+int n2wos_bvh_build_host(const float* verts, int nv, const uint32_t* tris, int nt, int leaf_size,
+                         void* nodes_out, int64_t* node_count, void* tris_out, float* prune_margin);
+/* Host copies of the device arrays: nodes (128 B each, node_count of them)
+ * and triangle records (48 B each).  Either pointer may be NULL. */
+int n2wos_bvh_export(const n2wos_bvh* bvh, void* nodes_out, void* tris_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* N2WOS_AMD_H */

@@ -1,0 +1,224 @@
+// bvh.hip -- BVH handle and the closest-point query kernels behind
+// n2wos_bvh_query_device[_masked|_counted].
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+
+#include "api_internal.hpp"
+#include "traverse.hpp"
+
+namespace n2 {
+
+// LDS ring capacity (entries per lane) of the stand-alone query kernels.
+constexpr int kQueryCap = 16;
+
+template <bool MASKED, bool COUNT>
+__global__ void __launch_bounds__(1024) k_closest_point(const Bvh4Node* __restrict__ nodes,
+                                                        const TriRec* __restrict__ tris, float delta,
+                                                        const float* __restrict__ points,
+                                                        const uint8_t* __restrict__ active, int n,
+                                                        float* __restrict__ out_d2, float* __restrict__ out_cp,
+                                                        int* __restrict__ out_tri, int* __restrict__ out_overflow,
+                                                        uint32_t* __restrict__ out_nodes,
+                                                        uint32_t* __restrict__ out_tris) {
+  extern __shared__ uint32_t lds[];
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const V3 p = v3(points[3 * q], points[3 * q + 1], points[3 * q + 2]);
+  if (MASKED && !active[q]) {
+    // inactive slot: neutral outputs (src/cuda/cubql_bvh.cu:198-204)
+    out_d2[q] = 0.0f;
+    out_cp[3 * q] = p.x; out_cp[3 * q + 1] = p.y; out_cp[3 * q + 2] = p.z;
+    out_tri[q] = -1;
+    out_overflow[q] = 0;
+    if (COUNT) { out_nodes[q] = 0; out_tris[q] = 0; }
+    return;
+  }
+  Hit h;
+  if (finite3(p)) {
+    uint32_t* sref = lds + threadIdx.x;
+    float* sd2 = reinterpret_cast<float*>(lds + kQueryCap * blockDim.x) + threadIdx.x;
+    h = bvh4_closest<kQueryCap, COUNT>(nodes, tris, p, delta, FLT_MAX, sref, sd2, blockDim.x);
+  } else {
+    h.d2 = FLT_MAX; h.cp = v3(0.0f, 0.0f, 0.0f); h.id = -1; h.overflow = 0; h.nodes = 0; h.tris = 0;
+  }
+  out_d2[q] = h.d2;
+  out_cp[3 * q] = h.cp.x; out_cp[3 * q + 1] = h.cp.y; out_cp[3 * q + 2] = h.cp.z;
+  out_tri[q] = h.id;
+  out_overflow[q] = h.overflow;
+  if (COUNT) { out_nodes[q] = h.nodes; out_tris[q] = h.tris; }
+}
+
+namespace {
+
+template <bool MASKED, bool COUNT>
+int launch_query(const n2wos_bvh* bvh, const float* d_points, const uint8_t* d_active, int n, float* d_d2,
+                 float* d_cp, int* d_tri, int* d_overflow, uint32_t* d_nodes, uint32_t* d_tris, int block,
+                 void* stream) {
+  N2_REQUIRE(bvh != nullptr, N2WOS_ERR_NULL, "bvh handle is NULL");
+  N2_REQUIRE(bvh->d_nodes && bvh->d_tris, N2WOS_ERR_EMPTY, "query_device called on an empty BVH");
+  N2_REQUIRE(n >= 0, N2WOS_ERR_INVALID, "query_count must be non-negative");
+  if (n == 0) return N2WOS_OK;
+  N2_REQUIRE(d_points && d_d2 && d_cp && d_tri && d_overflow, N2WOS_ERR_NULL,
+             "query_device received a null device pointer");
+  N2_REQUIRE(!MASKED || d_active, N2WOS_ERR_NULL, "query_device_masked received a null mask");
+  N2_REQUIRE(!COUNT || (d_nodes && d_tris), N2WOS_ERR_NULL, "counted query received a null counter pointer");
+  N2_REQUIRE(block >= 1 && block <= 1024, N2WOS_ERR_BLOCK, "block size must be in [1, 1024]");
+  const size_t lds = static_cast<size_t>(2 * kQueryCap) * static_cast<size_t>(block) * sizeof(uint32_t);
+  auto kern = k_closest_point<MASKED, COUNT>;
+  if (lds > 65536) {
+    N2_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(lds)));
+  }
+  const int grid = (n + block - 1) / block;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, static_cast<hipStream_t>(stream), bvh->d_nodes,
+                     bvh->d_tris, bvh->delta, d_points, d_active, n, d_d2, d_cp, d_tri, d_overflow, d_nodes,
+                     d_tris);
+  N2_HIP(hipGetLastError());
+  return N2WOS_OK;
+}
+
+}  // namespace
+}  // namespace n2
+
+using namespace n2;
+
+extern "C" int n2wos_bvh_create(const float* verts, int nv, const uint32_t* tris, int nt, int leaf_size,
+                                const char* build_method, n2wos_bvh** out) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(out != nullptr, N2WOS_ERR_NULL, "out handle pointer is NULL");
+    *out = nullptr;
+    N2_REQUIRE(verts && tris, N2WOS_ERR_NULL, "mesh arrays must not be NULL");
+    N2_REQUIRE(nv > 0 && nt > 0, N2WOS_ERR_EMPTY, "mesh has no vertices or no triangles");
+    std::string method = build_method ? build_method : "median";
+    if (method == "custom" || method == "custom_cuda_bvh") method = "median";
+    N2_REQUIRE(method == "median", N2WOS_ERR_UNSUPPORTED, "unsupported BVH build method: " + method);
+    const auto t0 = std::chrono::steady_clock::now();
+    HostBvh4 host = build_bvh4(verts, nv, tris, nt, leaf_size);
+    const auto t1 = std::chrono::steady_clock::now();
+    auto* h = new n2wos_bvh();
+    try {
+      N2_HIP(hipGetDevice(&h->device));
+      N2_HIP(hipMalloc(&h->d_nodes, host.nodes.size() * sizeof(Bvh4Node)));
+      N2_HIP(hipMalloc(&h->d_tris, host.tris.size() * sizeof(TriRec)));
+      N2_HIP(hipMemcpy(h->d_nodes, host.nodes.data(), host.nodes.size() * sizeof(Bvh4Node), hipMemcpyHostToDevice));
+      N2_HIP(hipMemcpy(h->d_tris, host.tris.data(), host.tris.size() * sizeof(TriRec), hipMemcpyHostToDevice));
+    } catch (...) {
+      if (h->d_nodes) (void)hipFree(h->d_nodes);
+      if (h->d_tris) (void)hipFree(h->d_tris);
+      delete h;
+      throw;
+    }
+    h->triangle_count = nt;
+    h->node_count = static_cast<int64_t>(host.nodes.size());
+    h->binary_nodes = host.binary_nodes;
+    h->leaf_size = host.leaf_size;
+    h->max_depth = host.binary_depth;
+    h->depth = host.depth;
+    h->delta = host.delta;
+    h->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    h->build_method = method;
+    const Bvh4Node& r = host.nodes[0];
+    for (int a = 0; a < 3; ++a) {
+      const float* lo = a == 0 ? r.lox : (a == 1 ? r.loy : r.loz);
+      const float* hi = a == 0 ? r.hix : (a == 1 ? r.hiy : r.hiz);
+      float l = lo[0], u = hi[0];
+      for (int k = 1; k < 4; ++k) {
+        l = std::min(l, lo[k]);
+        u = std::max(u, hi[k]);
+      }
+      h->bounds_lo[a] = l;
+      h->bounds_hi[a] = u;
+    }
+    *out = h;
+    return N2WOS_OK;
+  });
+}
+
+extern "C" int n2wos_bvh_destroy(n2wos_bvh* bvh) {
+  return guarded([&]() -> int {
+    if (!bvh) return N2WOS_OK;
+    if (bvh->d_nodes) (void)hipFree(bvh->d_nodes);
+    if (bvh->d_tris) (void)hipFree(bvh->d_tris);
+    delete bvh;
+    return N2WOS_OK;
+  });
+}
+
+extern "C" int n2wos_bvh_get_stats(const n2wos_bvh* bvh, n2wos_bvh_stats* out) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(bvh && out, N2WOS_ERR_NULL, "NULL argument");
+    std::memset(out, 0, sizeof(*out));
+    out->triangle_count = bvh->triangle_count;
+    out->node_count = bvh->node_count;
+    out->index_count = bvh->triangle_count;
+    out->binary_nodes = bvh->binary_nodes;
+    out->leaf_size = bvh->leaf_size;
+    out->max_depth = bvh->max_depth;
+    out->depth4 = bvh->depth;
+    out->prune_margin = bvh->delta;
+    out->build_milliseconds = bvh->build_ms;
+    out->device_bytes = bvh->node_count * static_cast<int64_t>(sizeof(Bvh4Node)) +
+                        bvh->triangle_count * static_cast<int64_t>(sizeof(TriRec));
+    for (int a = 0; a < 3; ++a) {
+      out->bounds_lo[a] = bvh->bounds_lo[a];
+      out->bounds_hi[a] = bvh->bounds_hi[a];
+    }
+    std::strncpy(out->build_method, bvh->build_method.c_str(), sizeof(out->build_method) - 1);
+    return N2WOS_OK;
+  });
+}
+
+extern "C" int n2wos_bvh_build_host(const float* verts, int nv, const uint32_t* tris, int nt, int leaf_size,
+                                    void* nodes_out, int64_t* node_count, void* tris_out, float* prune_margin) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(verts && tris && node_count, N2WOS_ERR_NULL, "NULL argument");
+    N2_REQUIRE(nv > 0 && nt > 0, N2WOS_ERR_EMPTY, "mesh has no vertices or no triangles");
+    HostBvh4 host = build_bvh4(verts, nv, tris, nt, leaf_size);
+    *node_count = static_cast<int64_t>(host.nodes.size());
+    if (prune_margin) *prune_margin = host.delta;
+    if (nodes_out) std::memcpy(nodes_out, host.nodes.data(), host.nodes.size() * sizeof(Bvh4Node));
+    if (tris_out) std::memcpy(tris_out, host.tris.data(), host.tris.size() * sizeof(TriRec));
+    return N2WOS_OK;
+  });
+}
+
+extern "C" int n2wos_bvh_export(const n2wos_bvh* bvh, void* nodes_out, void* tris_out) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(bvh != nullptr, N2WOS_ERR_NULL, "bvh handle is NULL");
+    if (nodes_out)
+      N2_HIP(hipMemcpy(nodes_out, bvh->d_nodes, bvh->node_count * sizeof(Bvh4Node), hipMemcpyDeviceToHost));
+    if (tris_out)
+      N2_HIP(hipMemcpy(tris_out, bvh->d_tris, bvh->triangle_count * sizeof(TriRec), hipMemcpyDeviceToHost));
+    return N2WOS_OK;
+  });
+}
+
+extern "C" int n2wos_bvh_query_device(const n2wos_bvh* bvh, const float* d_points, int query_count,
+                                      float* d_distance2, float* d_closest, int* d_triangle_id, int* d_overflow,
+                                      int block_size, void* stream) {
+  return guarded([&]() -> int {
+    return launch_query<false, false>(bvh, d_points, nullptr, query_count, d_distance2, d_closest, d_triangle_id,
+                                      d_overflow, nullptr, nullptr, block_size, stream);
+  });
+}
+
+extern "C" int n2wos_bvh_query_device_masked(const n2wos_bvh* bvh, const float* d_points, const uint8_t* d_active,
+                                             int query_count, float* d_distance2, float* d_closest,
+                                             int* d_triangle_id, int* d_overflow, int block_size, void* stream) {
+  return guarded([&]() -> int {
+    return launch_query<true, false>(bvh, d_points, d_active, query_count, d_distance2, d_closest, d_triangle_id,
+                                     d_overflow, nullptr, nullptr, block_size, stream);
+  });
+}
+
+extern "C" int n2wos_bvh_query_device_counted(const n2wos_bvh* bvh, const float* d_points, int query_count,
+                                              float* d_distance2, float* d_closest, int* d_triangle_id,
+                                              int* d_overflow, uint32_t* d_node_visits, uint32_t* d_tri_visits,
+                                              int block_size, void* stream) {
+  return guarded([&]() -> int {
+    return launch_query<false, true>(bvh, d_points, nullptr, query_count, d_distance2, d_closest, d_triangle_id,
+                                     d_overflow, d_node_visits, d_tri_visits, block_size, stream);
+  });
+}

@@ -1,0 +1,163 @@
+// traverse.hpp -- exact closest-point traversal of the 4-wide BVH on gfx950.
+//
+// One lane = one query.  The traversal stack holds (node ref, squared box
+// distance) pairs.  Its top CAP entries live in an LDS ring laid out
+// [slot][lane], so the 64 lanes of a wave always hit 64 different banks; when
+// a push meets a full ring, the ring's oldest entry moves to a per-lane spill
+// array (scratch memory, touched only by deep traversals).  Pops take the
+// ring first, then the spill array, so the visit order is exactly that of one
+// unbounded stack of CAP + kSpill entries; 3 x (4-wide depth) <= that total is
+// checked when the BVH is built, so the overflow flag stays 0.  Children are
+// visited nearest first; a popped entry whose stored distance already exceeds
+// the pruning threshold is discarded without touching its node.  Ties in
+// squared distance go to the lowest original triangle index, which
+// reproduces the brute-force scan of the reference oracle
+// (src/geometry/closest_point.cpp:50-68) exactly.
+#pragma once
+
+#include "bvh.hpp"
+#include "geom.hpp"
+
+namespace n2 {
+
+struct Hit {
+  float d2;
+  V3 cp;
+  int id;
+  int overflow;
+  uint32_t nodes;  // node lines loaded (counting variant only)
+  uint32_t tris;   // triangle records tested (counting variant only)
+};
+
+__device__ __forceinline__ void cswap(float& ka, uint32_t& ra, int& sa, float& kb, uint32_t& rb, int& sb) {
+  const bool sw = (kb < ka) || (kb == ka && sb < sa);
+  if (sw) {
+    const float tk = ka; ka = kb; kb = tk;
+    const uint32_t tr = ra; ra = rb; rb = tr;
+    const int ts = sa; sa = sb; sb = ts;
+  }
+}
+
+constexpr int kSpill = 48;
+
+// sref / sd2 point at this lane's column of the LDS ring; consecutive slots
+// are `stride` elements apart.  thr_init bounds the search from above (a
+// squared distance known to be >= the answer); FLT_MAX for an unbounded query.
+template <int CAP, bool COUNT>
+__device__ __forceinline__ Hit bvh4_closest(const Bvh4Node* __restrict__ nodes, const TriRec* __restrict__ tris,
+                                            V3 p, float delta, float thr_init, uint32_t* sref, float* sd2,
+                                            int stride) {
+  static_assert((CAP & (CAP - 1)) == 0, "ring capacity must be a power of two");
+  Hit h;
+  h.d2 = FLT_MAX;
+  h.cp = v3(0.0f, 0.0f, 0.0f);
+  h.id = -1;
+  h.overflow = 0;
+  h.nodes = 0;
+  h.tris = 0;
+  uint32_t spill_ref[kSpill];
+  float spill_d2[kSpill];
+  float thr = thr_init;
+  uint32_t cur = 0;
+  bool have = true;
+  int top = 0, cnt = 0, scnt = 0;
+  while (true) {
+    if (have) {
+      if (!(cur & kLeafFlag)) {
+        const float4* np = reinterpret_cast<const float4*>(nodes + cur);
+        const float4 lox = np[0], loy = np[1], loz = np[2];
+        const float4 hix = np[3], hiy = np[4], hiz = np[5];
+        const uint4 ch = reinterpret_cast<const uint4*>(np)[6];
+        if (COUNT) ++h.nodes;
+        float k0 = box_dist2(p, lox.x, loy.x, loz.x, hix.x, hiy.x, hiz.x);
+        float k1 = box_dist2(p, lox.y, loy.y, loz.y, hix.y, hiy.y, hiz.y);
+        float k2 = box_dist2(p, lox.z, loy.z, loz.z, hix.z, hiy.z, hiz.z);
+        float k3 = box_dist2(p, lox.w, loy.w, loz.w, hix.w, hiy.w, hiz.w);
+        const int nvalid = (k0 <= thr) + (k1 <= thr) + (k2 <= thr) + (k3 <= thr);
+        if (nvalid == 0) {
+          have = false;
+        } else {
+          uint32_t r0 = ch.x, r1 = ch.y, r2 = ch.z, r3 = ch.w;
+          int s0 = 0, s1 = 1, s2 = 2, s3 = 3;
+          k0 = k0 <= thr ? k0 : INFINITY;
+          k1 = k1 <= thr ? k1 : INFINITY;
+          k2 = k2 <= thr ? k2 : INFINITY;
+          k3 = k3 <= thr ? k3 : INFINITY;
+          cswap(k0, r0, s0, k1, r1, s1);
+          cswap(k2, r2, s2, k3, r3, s3);
+          cswap(k0, r0, s0, k2, r2, s2);
+          cswap(k1, r1, s1, k3, r3, s3);
+          cswap(k1, r1, s1, k2, r2, s2);
+          // push farthest first, keep descending into the nearest
+#define N2_PUSH(R, K)                                  \
+  do {                                                 \
+    if (cnt == CAP) {                                  \
+      if (scnt < kSpill) {                             \
+        spill_ref[scnt] = sref[top * stride];          \
+        spill_d2[scnt] = sd2[top * stride];            \
+        ++scnt;                                        \
+      } else {                                         \
+        h.overflow = 1;                                \
+      }                                                \
+    } else {                                           \
+      ++cnt;                                           \
+    }                                                  \
+    sref[top * stride] = (R);                          \
+    sd2[top * stride] = (K);                           \
+    top = (top + 1) & (CAP - 1);                       \
+  } while (0)
+          if (nvalid >= 4) N2_PUSH(r3, k3);
+          if (nvalid >= 3) N2_PUSH(r2, k2);
+          if (nvalid >= 2) N2_PUSH(r1, k1);
+#undef N2_PUSH
+          cur = r0;
+          continue;
+        }
+      } else {
+        const int first = static_cast<int>((cur >> 5) & 0x3ffffffu);
+        const int count = static_cast<int>(cur & 31u);
+        for (int i = 0; i < count; ++i) {
+          const float4* tp = reinterpret_cast<const float4*>(tris + first + i);
+          const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+          if (COUNT) ++h.tris;
+          const V3 a = v3(t0.x, t0.y, t0.z), b = v3(t0.w, t1.x, t1.y), c = v3(t1.z, t1.w, t2.x);
+          const int id = __float_as_int(t2.y);
+          const V3 q = closest_on_triangle(p, a, b, c);
+          const V3 e = q - p;
+          const float d2 = dot(e, e);
+          if (d2 < h.d2 || (d2 == h.d2 && id < h.id)) {
+            h.d2 = d2;
+            h.cp = q;
+            h.id = id;
+            thr = fminf(thr, prune_threshold(d2, delta));
+          }
+        }
+        have = false;
+      }
+    }
+    // pop the next entry that can still hold a closer triangle
+    while (cnt > 0 || scnt > 0) {
+      float k;
+      uint32_t r;
+      if (cnt > 0) {
+        top = (top - 1) & (CAP - 1);
+        --cnt;
+        k = sd2[top * stride];
+        r = sref[top * stride];
+      } else {
+        --scnt;
+        k = spill_d2[scnt];
+        r = spill_ref[scnt];
+      }
+      if (k <= thr) {
+        cur = r;
+        have = true;
+        break;
+      }
+    }
+    if (!have) break;
+  }
+  return h;
+}
+
+}  // namespace n2

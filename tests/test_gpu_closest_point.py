@@ -1,0 +1,109 @@
+"""GPU parity of the closest-point query API against the oracle (bit-exact)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.fixture(scope="module")
+def c1(eng, orc, torch_gpu):
+    v, t = eng.uv_sphere(64, 128, 1, [0.1])
+    vn, _, _ = eng.normalize_to_unit_radius(v)
+    bvh = eng.Bvh(vn, t, leaf_size=8)
+    return vn, t, bvh
+
+
+def test_c1_queries_bit_exact_vs_bruteforce(eng, orc, torch_gpu, c1):
+    vn, t, bvh = c1
+    torch = torch_gpu
+    rng = np.random.default_rng(0)
+    pts = rng.uniform(-1.2, 1.2, (65536, 3)).astype(np.float32)
+    print("case: c1 uniform 65536 queries", flush=True)
+    d2, cp, tri, ovf = bvh.query(pts)
+    sub = slice(0, 8192)
+    d2b, cpb, trb = orc.bruteforce(vn, t, pts[sub], nthreads=16)
+    assert np.array_equal(bits(d2[sub]), bits(d2b))
+    assert np.array_equal(tri[sub], trb)
+    assert np.array_equal(bits(cp[sub]), bits(cpb))
+    # the whole set against the oracle's traversal restatement (same results, exact)
+    nodes, recs, delta = bvh.export()
+    d2o, cpo, trio, ovfo, nno, nto = orc.bvh4_query(nodes, recs, delta, pts)
+    assert np.array_equal(bits(d2), bits(d2o)) and np.array_equal(tri, trio) and np.array_equal(bits(cp), bits(cpo))
+    assert ovf.sum() == 0
+
+
+def test_counted_query_matches_restated_visit_counts(eng, orc, torch_gpu, c1):
+    vn, t, bvh = c1
+    torch = torch_gpu
+    rng = np.random.default_rng(1)
+    pts = np.concatenate([rng.uniform(-1.2, 1.2, (4096, 3)), rng.uniform(-0.02, 0.02, (256, 3))]).astype(np.float32)
+    print("case: counted query 4352", flush=True)
+    d2, cp, tri, ovf, nn, nt = bvh.query_tensors(torch.from_numpy(pts).cuda(), counted=True)
+    torch.cuda.synchronize()
+    nodes, recs, delta = bvh.export()
+    d2o, cpo, trio, ovfo, nno, nto = orc.bvh4_query(nodes, recs, delta, pts)
+    assert np.array_equal(nn.cpu().numpy().astype(np.uint32), nno)
+    assert np.array_equal(nt.cpu().numpy().astype(np.uint32), nto)
+    assert np.array_equal(bits(d2.cpu().numpy()), bits(d2o))
+
+
+def test_near_surface_and_far_points(eng, orc, torch_gpu, c1):
+    vn, t, bvh = c1
+    rng = np.random.default_rng(2)
+    base = rng.uniform(-1.2, 1.2, (2000, 3)).astype(np.float32)
+    _, cpb, _ = orc.bruteforce(vn, t, base, nthreads=16)
+    pts = np.concatenate([cpb + rng.normal(0, 1e-5, cpb.shape), cpb, rng.uniform(-50, 50, (500, 3)),
+                          vn[:300]]).astype(np.float32)
+    print("case: near-surface/far/vertex queries", len(pts), flush=True)
+    d2, cp, tri, ovf = bvh.query(pts)
+    d2b, cpb2, trb = orc.bruteforce(vn, t, pts, nthreads=16)
+    assert np.array_equal(bits(d2), bits(d2b)) and np.array_equal(tri, trb) and np.array_equal(bits(cp), bits(cpb2))
+
+
+def test_masked_and_nonfinite_and_block_sizes(eng, orc, torch_gpu, c1):
+    vn, t, bvh = c1
+    torch = torch_gpu
+    rng = np.random.default_rng(3)
+    pts = rng.uniform(-1.2, 1.2, (1000, 3)).astype(np.float32)
+    pts[5] = [np.nan, 0, 0]
+    pts[6] = [np.inf, 0, 0]
+    active = (rng.uniform(size=1000) < 0.5).astype(np.uint8)
+    active[5] = active[6] = 1
+    print("case: masked query 1000", flush=True)
+    tp = torch.from_numpy(pts).cuda()
+    ta = torch.from_numpy(active).cuda()
+    d2, cp, tri, ovf = [x.cpu().numpy() for x in bvh.query_tensors(tp, active=ta)]
+    torch.cuda.synchronize()
+    full = [x.cpu().numpy() for x in bvh.query_tensors(tp)]
+    on = active == 1
+    assert np.array_equal(bits(d2[on]), bits(full[0][on])) and np.array_equal(tri[on], full[2][on])
+    off = ~on
+    assert np.all(d2[off] == 0) and np.all(tri[off] == -1) and np.array_equal(cp[off], pts[off])
+    assert d2[5] == np.finfo(np.float32).max and tri[5] == -1 and d2[6] == np.finfo(np.float32).max
+    for bs in (1, 64, 256, 1024):
+        print("case: block size", bs, flush=True)
+        r = [x.cpu().numpy() for x in bvh.query_tensors(tp, block_size=bs)]
+        assert np.array_equal(bits(r[0]), bits(full[0])) and np.array_equal(r[2], full[2])
+
+
+def test_query_errors(eng, torch_gpu, c1):
+    vn, t, bvh = c1
+    torch = torch_gpu
+    x = torch.zeros((4, 3), device="cuda")
+    d2 = torch.zeros(4, device="cuda")
+    with pytest.raises(eng.N2wosError) as e:
+        bvh.query_device(x.data_ptr(), -1, d2.data_ptr(), x.data_ptr(), d2.data_ptr(), d2.data_ptr())
+    assert e.value.code == eng._lib.ERR_INVALID
+    bvh.query_device(x.data_ptr(), 0, 0, 0, 0, 0)  # no-op
+    with pytest.raises(eng.N2wosError) as e:
+        bvh.query_device(x.data_ptr(), 4, d2.data_ptr(), x.data_ptr(), d2.data_ptr(), d2.data_ptr(), block_size=2048)
+    assert e.value.code == eng._lib.ERR_BLOCK
+    with pytest.raises(eng.N2wosError) as e:
+        bvh.query_device(x.data_ptr(), 4, 0, x.data_ptr(), d2.data_ptr(), d2.data_ptr())
+    assert e.value.code == eng._lib.ERR_NULL
+    st = bvh.stats()
+    assert st["triangle_count"] == len(t) and st["leaf_size"] == 8 and st["build_method"] == "median"

@@ -1,0 +1,213 @@
+"""The oracle pinned: against the reference's own compiled CPU core, committed
+golden vectors, published known-answer tests and closed forms."""
+import ctypes as C
+import hashlib
+import math
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "closest_point_ref.npz")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return np.load(GOLDEN, allow_pickle=False)
+
+
+@pytest.fixture(scope="module")
+def c1_mesh(orc):
+    v, t = orc.uv_sphere(64, 128, 1, [0.1])
+    vn, c, s = orc.normalize(v)
+    return vn, t, c, s
+
+
+def test_reference_bumpy_sphere_matches_golden_hash(orc, golden):
+    v, t = orc.uv_sphere(64, 128, 0, [0.15])
+    vn, c, s = orc.normalize(v)
+    assert hashlib.sha256(vn.astype("<f4").tobytes()).hexdigest() == str(golden["mesh_ref_verts_sha256"])
+    assert hashlib.sha256(t.astype("<u4").tobytes()).hexdigest() == str(golden["mesh_ref_tris_sha256"])
+    assert np.array_equal(bits(c), bits(golden["center"]))
+    assert np.float32(s) == golden["scale"][0]
+
+
+def test_sin6_normalisation_matches_golden(orc, golden, c1_mesh):
+    vn, t, c, s = c1_mesh
+    assert np.array_equal(bits(c), bits(golden["sin6_center"]))
+    assert np.float32(s) == golden["sin6_scale"]
+
+
+def test_bruteforce_matches_reference_golden_bit_exact(orc, golden, c1_mesh):
+    vn, t, _, _ = c1_mesh
+    d2, cp, tri = orc.bruteforce(vn, t, golden["pts"], nthreads=4)
+    assert np.array_equal(bits(d2), bits(golden["d2"]))
+    assert np.array_equal(bits(cp), bits(golden["cp"]))
+    assert np.array_equal(tri, golden["tri"])
+
+
+def test_closest_point_on_triangle_golden_cases(orc, golden):
+    for row in golden["tri_cases"]:
+        p, a, b, c, want = row[0:3], row[3:6], row[6:9], row[9:12], row[12:15]
+        assert np.array_equal(bits(orc.closest_point_on_triangle(p, a, b, c)), bits(want))
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(os.path.dirname(__file__)), "oracle", "_ref",
+                                                    "libn2wos_ref.so")), reason="reference core not built here")
+def test_oracle_against_compiled_reference_core(orc):
+    R = orc.ref()
+    for stacks, slices, amp in [(4, 8, 0.0), (16, 32, 0.5), (64, 128, 0.15), (33, 77, 0.3)]:
+        nv, nt = C.c_int(), C.c_int()
+        R.ref_bumpy_sphere(stacks, slices, amp, 1, C.byref(nv), C.byref(nt), None, None, None, None)
+        rv = np.zeros((nv.value, 3), np.float32)
+        rt = np.zeros((nt.value, 3), np.uint32)
+        rc = np.zeros(3, np.float32)
+        rs = np.zeros(1, np.float32)
+        R.ref_bumpy_sphere(stacks, slices, amp, 1, C.byref(nv), C.byref(nt), rv.ctypes.data, rt.ctypes.data,
+                           rc.ctypes.data, rs.ctypes.data)
+        v, t = orc.uv_sphere(stacks, slices, 0, [amp])
+        vn, c, s = orc.normalize(v)
+        assert np.array_equal(t, rt) and np.array_equal(bits(vn), bits(rv)) and np.float32(s) == rs[0]
+        rng = np.random.default_rng(stacks)
+        pts = rng.uniform(-1.5, 1.5, (300, 3)).astype(np.float32)
+        d2r = np.zeros(300, np.float32)
+        cpr = np.zeros((300, 3), np.float32)
+        trr = np.zeros(300, np.int32)
+        R.ref_bruteforce(vn, len(vn), t, len(t), pts, 300, d2r, cpr, trr)
+        d2, cp, tri = orc.bruteforce(vn, t, pts)
+        assert np.array_equal(bits(d2), bits(d2r)) and np.array_equal(bits(cp), bits(cpr))
+        assert np.array_equal(tri, trr)
+        assert orc.lib().orc_count_degenerate(vn, t, len(t), 1e-20) == R.ref_count_degenerate(vn, len(vn), t, len(t),
+                                                                                               1e-20)
+
+
+def test_triangle_regions_closed_form(orc):
+    a, b, c = np.array([0, 0, 0.]), np.array([2, 0, 0.]), np.array([0, 2, 0.])
+    # face interior: projection, distance = |z|
+    assert np.allclose(orc.closest_point_on_triangle([0.5, 0.5, 3.0], a, b, c), [0.5, 0.5, 0.0])
+    # vertex regions
+    assert np.allclose(orc.closest_point_on_triangle([-1, -1, 1], a, b, c), a)
+    assert np.allclose(orc.closest_point_on_triangle([3, -1, 0], a, b, c), b)
+    assert np.allclose(orc.closest_point_on_triangle([-1, 3, 0], a, b, c), c)
+    # edge regions: ab, ac, bc
+    assert np.allclose(orc.closest_point_on_triangle([1, -1, 0], a, b, c), [1, 0, 0])
+    assert np.allclose(orc.closest_point_on_triangle([-1, 1, 0], a, b, c), [0, 1, 0])
+    assert np.allclose(orc.closest_point_on_triangle([2, 2, 0], a, b, c), [1, 1, 0])
+    # degenerate (collinear) triangle: result lies on the segment
+    q = orc.closest_point_on_triangle([0.5, 1.0, 0.0], [0, 0, 0.], [1, 0, 0.], [2, 0, 0.])
+    assert abs(q[1]) < 1e-7 and 0.0 <= q[0] <= 2.0
+
+
+def test_closest_point_not_farther_than_any_sample(orc):
+    """Brute-force invariant on tiny inputs: the returned point is on the
+    triangle (barycentric) and no sampled triangle point is closer."""
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        p, a, b, c = rng.normal(size=(4, 3)).astype(np.float32)
+        q = orc.closest_point_on_triangle(p, a, b, c).astype(np.float64)
+        u, v = np.meshgrid(np.linspace(0, 1, 41), np.linspace(0, 1, 41))
+        m = (u + v) <= 1
+        s = a[None] + u[m][:, None] * (b - a)[None] + v[m][:, None] * (c - a)[None]
+        dmin = np.sqrt(((s - p) ** 2).sum(1)).min()
+        assert np.linalg.norm(q - p) <= dmin + 1e-5
+
+
+def test_bruteforce_lowest_index_wins_ties(orc):
+    # the same triangle twice: the tie goes to index 0; a permuted copy keeps it
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    t = np.array([[0, 1, 2], [0, 1, 2], [1, 2, 0]], np.uint32)
+    d2, cp, tri = orc.bruteforce(v, t, [[0.2, 0.2, 1.0]])
+    assert tri[0] == 0
+
+
+def test_philox_known_answers(orc):
+    # Random123 kat_vectors for philox4x32_10
+    assert [hex(x) for x in orc.philox([0, 0, 0, 0], [0, 0])] == ["0x6627e8d5", "0xe169c58d", "0xbc57ac4c",
+                                                                   "0x9b00dbd8"]
+    assert [hex(x) for x in orc.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2)] == ["0x408f276d", "0x41c83b0e",
+                                                                                "0xa20bc7c6", "0x6d5451fd"]
+    assert [hex(x) for x in orc.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344],
+                                       [0xA4093822, 0x299F31D0])] == ["0xd16cfe09", "0x94fdcceb", "0x5001e420",
+                                                                      "0x24126ea1"]
+
+
+def test_u01_open_interval_and_exact(orc):
+    L = orc.lib()
+    assert L.orc_u01(0) == 2.0 ** -24
+    assert L.orc_u01(0xFFFFFFFF) == 1.0 - 2.0 ** -24
+    assert 0.0 < L.orc_u01(0x12345678) < 1.0
+
+
+def test_sincos2pi_accuracy(orc):
+    vs = np.concatenate([np.linspace(-2, 2, 4001), np.random.default_rng(1).uniform(0, 1, 4000)]).astype(np.float32)
+    worst = 0.0
+    for v in vs:
+        s, c = orc.sincos2pi(float(v))
+        ang = 2 * math.pi * float(v)
+        worst = max(worst, abs(s - math.sin(ang)), abs(c - math.cos(ang)))
+        assert abs(s * s + c * c - 1.0) < 5e-7
+    assert worst < 4e-7
+
+
+def test_sphere_direction_moments(orc):
+    """E[d] = 0 and E[d d^T] = I/3 within 5 sigma for the walk's direction map."""
+    n = 20000
+    dirs = np.zeros((n, 3))
+    L = orc.lib()
+    for i in range(n):
+        r = orc.philox([i, 0, 0, 0], [1, 2])
+        z = 1.0 - 2.0 * L.orc_u01(int(r[0]))
+        s, c = orc.sincos2pi(L.orc_u01(int(r[1])))
+        rr = math.sqrt(max(0.0, 1 - z * z))
+        dirs[i] = [rr * c, rr * s, z]
+    assert np.all(np.abs(dirs.mean(0)) < 5 * math.sqrt(1 / 3 / n))
+    cov = dirs.T @ dirs / n
+    assert np.all(np.abs(cov - np.eye(3) / 3) < 5 * 0.3 / math.sqrt(n))
+
+
+def test_spherical_harmonics_orthonormal(orc):
+    L = orc.lib()
+    th = (np.arange(60) + 0.5) * math.pi / 60
+    ph = (np.arange(120) + 0.5) * 2 * math.pi / 120
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    w = np.sin(T) * (math.pi / 60) * (2 * math.pi / 120)
+    lm = [(0, 0), (1, -1), (1, 0), (2, 1), (3, -2), (5, 4), (8, -8), (8, 3)]
+    Y = {k: np.vectorize(lambda t, p, k=k: L.orc_sh_real(k[0], k[1], t, p))(T, P) for k in lm}
+    for i, a in enumerate(lm):
+        for b in lm[i:]:
+            ip = (Y[a] * Y[b] * w).sum()
+            assert abs(ip - (1.0 if a == b else 0.0)) < 2e-3
+    assert abs(L.orc_sh_real(0, 0, 0.3, 0.2) - 0.5 / math.sqrt(math.pi)) < 1e-12
+    assert abs(L.orc_sh_real(1, 0, 0.3, 0.2) - math.sqrt(3 / (4 * math.pi)) * math.cos(0.3)) < 1e-12
+
+
+def test_bvh2_equals_bruteforce(orc, c1_mesh):
+    vn, t, _, _ = c1_mesh
+    rng = np.random.default_rng(3)
+    pts = rng.uniform(-1.2, 1.2, (3000, 3)).astype(np.float32)
+    b = orc.Bvh2(vn, t)
+    d2b, cpb, trb = orc.bruteforce(vn, t, pts, nthreads=4)
+    d2, cp, tri = b.query(pts, nthreads=4)
+    assert np.array_equal(bits(d2), bits(d2b)) and np.array_equal(tri, trb) and np.array_equal(bits(cp), bits(cpb))
+
+
+def test_welford_and_chan_against_two_pass(orc):
+    rng = np.random.default_rng(5)
+    x = rng.normal(3.0, 0.5, 1000).astype(np.float32)
+    m, s = C.c_double(), C.c_double()
+    orc.lib().orc_welford(x, len(x), C.byref(m), C.byref(s))
+    xd = x.astype(np.float64)
+    assert abs(m.value - xd.mean()) < 1e-12 and abs(s.value - ((xd - xd.mean()) ** 2).sum()) < 1e-9
+    out = np.zeros(3)
+    a, b = xd[:300], xd[300:]
+    orc.lib().orc_chan_merge(300, a.mean(), ((a - a.mean()) ** 2).sum(), 700, b.mean(), ((b - b.mean()) ** 2).sum(),
+                             out)
+    assert out[0] == 1000 and abs(out[1] - xd.mean()) < 1e-12 and abs(out[2] - s.value) < 1e-9
+    n, mean, m2 = orc.point_stats(x, 4, 250)
+    for p in range(4):
+        seg = xd[250 * p: 250 * (p + 1)]
+        assert n[p] == 250 and abs(mean[p] - seg.mean()) < 1e-12 and abs(m2[p] - ((seg - seg.mean()) ** 2).sum()) < 1e-9
