@@ -29,6 +29,13 @@
  *   n2wos_normalize_to_unit_radius   normalize_to_unit_radius (src/mesh/normalize.cpp:18-37)
  *   n2wos_count_degenerate_triangles count_degenerate_triangles
  *                                    (src/geometry/closest_point.cpp:70-83)
+ *   n2wos_bvh_inside_device          point_inside_closed_mesh_ray_x
+ *                                    (src/cuda/cubql_bvh.cu:407-455, eval_tcnn_nc_wos.cu:399-400)
+ *   n2wos_wos_solve                  run_persistent_harmonic (include/n2wos/wos_wavefront.hpp:120-122,
+ *                                    src/cuda/cubql_bvh.cu:819-960) and launch_nc_pure_wos_samples +
+ *                                    host summarize (include/n2wos/tcnn_nc_wos.hpp:79-134,
+ *                                    src/tools/eval_tcnn_nc_wos.cu:525-597): walks plus per-point
+ *                                    FP64 mean/variance on the device
  */
 #ifndef N2WOS_AMD_H
 #define N2WOS_AMD_H
@@ -58,6 +65,9 @@ typedef struct n2wos_bvh n2wos_bvh;
 /* Description of the last failure on the calling thread ("" if none). */
 const char* n2wos_last_error(void);
 int n2wos_abi_version(void);
+/* Launch tuning for benchmarks: key "ring" = LDS stack entries per lane of the
+ * walker kernel (8 or 16; default 8).  Process-wide. */
+int n2wos_set_tuning(const char* key, int value);
 /* Device name and runtime version of the current HIP device, for reports. */
 int n2wos_device_summary(char* buf, int buf_len);
 
@@ -135,6 +145,60 @@ int n2wos_bvh_build_host(const float* verts, int nv, const uint32_t* tris, int n
 /* Host copies of the device arrays: nodes (128 B each, node_count of them)
  * and triangle records (48 B each).  Either pointer may be NULL. */
 int n2wos_bvh_export(const n2wos_bvh* bvh, void* nodes_out, void* tris_out);
+
+/* Inside/outside classification of points against the (closed) mesh: ray
+ * parity along three fixed generic directions, majority vote.  d_inside[i]
+ * = 1 inside, 0 outside. */
+int n2wos_bvh_inside_device(const n2wos_bvh* bvh, const float* d_points, int n, uint8_t* d_inside, void* stream);
+
+/* -------------------------------------------------------- Walk-on-Spheres */
+/* Dirichlet boundary data g.  Modes 0, 3, 4, 5, 12 are the reference's
+ * NcBoundaryMode ids (include/n2wos/tcnn_nc_wos.hpp:15-36). */
+enum {
+  N2WOS_BC_X2_MINUS_Y2 = 0,
+  N2WOS_BC_ZEBRA_K4 = 3,
+  N2WOS_BC_ZEBRA_K8 = 4,
+  N2WOS_BC_ZEBRA_K12 = 5,
+  N2WOS_BC_CONSTANT_ONE = 12,
+  N2WOS_BC_CHARGES = 100,       /* g = sum_i q_i / |x - c_i| */
+  N2WOS_BC_POISSON_SINES = 101  /* g = u* = sin(2pi x) sin(2pi y) sin(2pi z) + |x|^2/2 */
+};
+typedef struct {
+  int mode;
+  int n_charges;          /* <= 16, for N2WOS_BC_CHARGES */
+  float charges[16][4];   /* cx, cy, cz, q */
+} n2wos_boundary;
+
+enum {
+  N2WOS_SOURCE_NONE = 0,         /* Laplace */
+  N2WOS_SOURCE_POISSON_SINES = 1 /* -Laplace u = f = 12 pi^2 sss - 3, Green's-function sampling */
+};
+typedef struct {
+  int max_steps;      /* reference default 256 */
+  float epsilon;      /* stop when d <= epsilon; reference default 1e-4 */
+  float step_scale;   /* R = step_scale * d; reference default 0.999 */
+  int source;         /* N2WOS_SOURCE_* */
+  uint64_t seed;      /* Philox key */
+  uint32_t rng_stream;/* Philox counter word 3 = 2 rng_stream (+1) */
+  int max_blocks;     /* 0 = fill the device */
+} n2wos_walk_params;
+
+/* Bytes of device workspace n2wos_wos_solve needs for this job. */
+int64_t n2wos_wos_workspace_size(int n_points, int walks_per_point);
+/* walks_per_point walks from each point; per-point FP64 (n, mean, M2),
+ * total steps and forced-at-max-steps counts.  With accumulate != 0 the new
+ * statistics are merged (Chan) into the existing contents of d_n, d_mean,
+ * d_m2, d_steps, d_forced; otherwise they are overwritten.  Point i uses RNG
+ * point id point_offset + i, so a job split into point ranges reproduces the
+ * unsplit job bit for bit.  d_walk_values / d_walk_steps (both NULL or both
+ * set, n_points * walks_per_point entries) receive each walk's value and step
+ * count.  No allocation: the caller passes a workspace of at least
+ * n2wos_wos_workspace_size bytes. */
+int n2wos_wos_solve(const n2wos_bvh* bvh, const n2wos_boundary* boundary, const n2wos_walk_params* params,
+                    const float* d_points, int n_points, int walks_per_point, uint32_t point_offset, double* d_n,
+                    double* d_mean, double* d_m2, uint64_t* d_steps, uint32_t* d_forced, int accumulate,
+                    void* d_workspace, int64_t workspace_bytes, float* d_walk_values, int* d_walk_steps,
+                    void* stream);
 
 #ifdef __cplusplus
 }

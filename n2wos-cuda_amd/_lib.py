@@ -58,6 +58,7 @@ f32 = C.c_float
 _SIGNATURES = {
     "n2wos_last_error": ([], C.c_char_p),
     "n2wos_abi_version": ([], i32),
+    "n2wos_set_tuning": ([C.c_char_p, i32], i32),
     "n2wos_device_summary": ([C.c_char_p, i32], i32),
     "n2wos_uv_sphere_counts": ([i32, i32, C.POINTER(i32), C.POINTER(i32)], i32),
     "n2wos_uv_sphere": ([i32, i32, i32, vp, i32, vp, vp], i32),
@@ -71,6 +72,9 @@ _SIGNATURES = {
     "n2wos_bvh_query_device_counted": ([vp, vp, i32, vp, vp, vp, vp, vp, vp, i32, vp], i32),
     "n2wos_bvh_build_host": ([vp, i32, vp, i32, i32, vp, C.POINTER(i64), vp, C.POINTER(f32)], i32),
     "n2wos_bvh_export": ([vp, vp, vp], i32),
+    "n2wos_bvh_inside_device": ([vp, vp, i32, vp, vp], i32),
+    "n2wos_wos_workspace_size": ([i32, i32], i64),
+    "n2wos_wos_solve": ([vp, vp, vp, vp, i32, i32, C.c_uint32, vp, vp, vp, vp, vp, i32, vp, i64, vp, vp, vp], i32),
 }
 
 

@@ -7,6 +7,7 @@
 
 #include "api_internal.hpp"
 #include "traverse.hpp"
+#include "wos.hpp"
 
 namespace n2 {
 
@@ -50,6 +51,65 @@ __global__ void __launch_bounds__(1024) k_closest_point(const Bvh4Node* __restri
   if (COUNT) { out_nodes[q] = h.nodes; out_tris[q] = h.tris; }
 }
 
+// Ray-parity inside test (Moller-Trumbore hits with t > 0 along three fixed
+// generic directions; majority vote), one lane per point.
+__device__ int ray_crossings(const Bvh4Node* __restrict__ nodes, const TriRec* __restrict__ tris, V3 o, V3 dir) {
+  const V3 inv = v3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+  uint32_t stack[64];
+  int sp = 0, hits = 0;
+  stack[sp++] = 0;
+  while (sp > 0) {
+    const uint32_t ref = stack[--sp];
+    if (!(ref & kLeafFlag)) {
+      const Bvh4Node& nd = nodes[ref];
+      for (int k = 0; k < 4; ++k) {
+        const float tx0 = (nd.lox[k] - o.x) * inv.x, tx1 = (nd.hix[k] - o.x) * inv.x;
+        const float ty0 = (nd.loy[k] - o.y) * inv.y, ty1 = (nd.hiy[k] - o.y) * inv.y;
+        const float tz0 = (nd.loz[k] - o.z) * inv.z, tz1 = (nd.hiz[k] - o.z) * inv.z;
+        const float tmin = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+        const float tmax = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+        const uint32_t c = nd.child[k];
+        if (tmin <= tmax && !((c & kLeafFlag) && (c & 31u) == 0u) && sp < 64) stack[sp++] = c;
+      }
+    } else {
+      const int first = static_cast<int>((ref >> 5) & 0x3ffffffu), count = static_cast<int>(ref & 31u);
+      for (int i = 0; i < count; ++i) {
+        const TriRec& t = tris[first + i];
+        const V3 a = v3(t.ax, t.ay, t.az);
+        const V3 e1 = v3(t.bx, t.by, t.bz) - a, e2 = v3(t.cx, t.cy, t.cz) - a;
+        const V3 h = v3(dir.y * e2.z - dir.z * e2.y, dir.z * e2.x - dir.x * e2.z, dir.x * e2.y - dir.y * e2.x);
+        const float det = dot(e1, h);
+        if (fabsf(det) < 1e-20f) continue;
+        const float inv_det = 1.0f / det;
+        const V3 s = o - a;
+        const float u = dot(s, h) * inv_det;
+        if (u < 0.0f || u > 1.0f) continue;
+        const V3 q = v3(s.y * e1.z - s.z * e1.y, s.z * e1.x - s.x * e1.z, s.x * e1.y - s.y * e1.x);
+        const float v = dot(dir, q) * inv_det;
+        if (v < 0.0f || u + v > 1.0f) continue;
+        if (dot(e2, q) * inv_det > 0.0f) ++hits;
+      }
+    }
+  }
+  return hits;
+}
+
+__global__ void k_inside(const Bvh4Node* __restrict__ nodes, const TriRec* __restrict__ tris,
+                         const float* __restrict__ points, int n, uint8_t* __restrict__ inside) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const V3 p = v3(points[3 * q], points[3 * q + 1], points[3 * q + 2]);
+  if (!finite3(p)) {
+    inside[q] = 0;
+    return;
+  }
+  const V3 dirs[3] = {v3(0.9234801f, 0.2901198f, 0.2510278f), v3(-0.3376619f, 0.8152146f, 0.4705419f),
+                      v3(0.1666621f, -0.2151597f, 0.9622526f)};
+  int votes = 0;
+  for (int k = 0; k < 3; ++k) votes += ray_crossings(nodes, tris, p, dirs[k]) & 1;
+  inside[q] = votes >= 2 ? 1 : 0;
+}
+
 namespace {
 
 template <bool MASKED, bool COUNT>
@@ -80,6 +140,12 @@ int launch_query(const n2wos_bvh* bvh, const float* d_points, const uint8_t* d_a
 }
 
 }  // namespace
+
+void launch_closest_point(const n2wos_bvh* bvh, const float* d_points, int n, float* d_d2, float* d_cp, int* d_tri,
+                          int* d_ovf, hipStream_t stream) {
+  launch_query<false, false>(bvh, d_points, nullptr, n, d_d2, d_cp, d_tri, d_ovf, nullptr, nullptr, 128, stream);
+}
+
 }  // namespace n2
 
 using namespace n2;
@@ -210,6 +276,22 @@ extern "C" int n2wos_bvh_query_device_masked(const n2wos_bvh* bvh, const float* 
   return guarded([&]() -> int {
     return launch_query<true, false>(bvh, d_points, d_active, query_count, d_distance2, d_closest, d_triangle_id,
                                      d_overflow, nullptr, nullptr, block_size, stream);
+  });
+}
+
+extern "C" int n2wos_bvh_inside_device(const n2wos_bvh* bvh, const float* d_points, int n, uint8_t* d_inside,
+                                       void* stream) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(bvh != nullptr, N2WOS_ERR_NULL, "bvh handle is NULL");
+    N2_REQUIRE(bvh->d_nodes && bvh->d_tris, N2WOS_ERR_EMPTY, "inside test on an empty BVH");
+    N2_REQUIRE(n >= 0, N2WOS_ERR_INVALID, "n must be non-negative");
+    if (n == 0) return N2WOS_OK;
+    N2_REQUIRE(d_points && d_inside, N2WOS_ERR_NULL, "NULL device pointer");
+    const int block = 128;
+    hipLaunchKernelGGL(k_inside, dim3((n + block - 1) / block), dim3(block), 0, static_cast<hipStream_t>(stream),
+                       bvh->d_nodes, bvh->d_tris, d_points, n, d_inside);
+    N2_HIP(hipGetLastError());
+    return N2WOS_OK;
   });
 }
 

@@ -11,6 +11,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "api_internal.hpp"
@@ -20,6 +22,17 @@ namespace n2 {
 thread_local std::string g_last_error;
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+std::mutex g_tuning_mu;
+std::map<std::string, int> g_tuning;
+}  // namespace
+
+int tuning_value(const char* key, int fallback) {
+  std::lock_guard<std::mutex> lock(g_tuning_mu);
+  const auto it = g_tuning.find(key);
+  return it == g_tuning.end() ? fallback : it->second;
+}
 
 namespace {
 
@@ -92,6 +105,18 @@ using namespace n2;
 extern "C" const char* n2wos_last_error(void) { return g_last_error.c_str(); }
 
 extern "C" int n2wos_abi_version(void) { return N2WOS_ABI_VERSION; }
+
+extern "C" int n2wos_set_tuning(const char* key, int value) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(key != nullptr, N2WOS_ERR_NULL, "NULL key");
+    const std::string k(key);
+    N2_REQUIRE(k == "ring", N2WOS_ERR_INVALID, "unknown tuning key: " + k);
+    N2_REQUIRE(value == 8 || value == 16, N2WOS_ERR_INVALID, "ring must be 8 or 16");
+    std::lock_guard<std::mutex> lock(g_tuning_mu);
+    g_tuning[k] = value;
+    return N2WOS_OK;
+  });
+}
 
 extern "C" int n2wos_device_summary(char* buf, int buf_len) {
   return guarded([&]() -> int {

@@ -1,0 +1,72 @@
+// wos.hpp -- argument blocks of the persistent walker.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "api_internal.hpp"
+#include "bvh.hpp"
+#include "geom.hpp"
+
+namespace n2 {
+
+struct WosKernelArgs {
+  const Bvh4Node* nodes;
+  const TriRec* tris;
+  float delta;
+  Boundary bc;
+  const float* points;
+  int walks_per_point;
+  int units_per_point;
+  int total_units;
+  int max_steps;
+  float epsilon;
+  float step_scale;
+  uint32_t seed_lo, seed_hi;
+  uint32_t stream;
+  uint32_t point_offset;
+  int* next_unit;
+  double* unit_mean;
+  double* unit_m2;
+  uint32_t* unit_steps;
+  uint32_t* unit_flags;  // forced count (low 16 bits) | overflow count << 16
+  float* walk_values;    // optional per-walk outputs
+  int* walk_steps;
+  const float* first_d2;  // first query of each point (shared by its walks)
+  const float* first_cp;
+  const int* first_ovf;
+};
+
+struct WosRun {
+  const float* d_points;
+  int n_points;
+  int walks_per_point;
+  uint32_t point_offset;
+  int max_steps;
+  float epsilon;
+  float step_scale;
+  uint64_t seed;
+  uint32_t rng_stream;
+  int source;
+  int max_blocks;
+  double* d_n;
+  double* d_mean;
+  double* d_m2;
+  uint64_t* d_steps;
+  uint32_t* d_forced;
+  int accumulate;
+  void* d_workspace;
+  float* d_walk_values;
+  int* d_walk_steps;
+};
+
+int64_t wos_workspace_bytes(int n_points, int walks_per_point);
+// Enqueues the stand-alone closest-point kernel (bvh.hip).
+void launch_closest_point(const n2wos_bvh* bvh, const float* d_points, int n, float* d_d2, float* d_cp, int* d_tri,
+                          int* d_ovf, hipStream_t stream);
+// Tuning knobs set through n2wos_set_tuning (default when unset).
+int tuning_value(const char* key, int fallback);
+void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStream_t stream);
+
+}  // namespace n2
