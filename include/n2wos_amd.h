@@ -65,8 +65,10 @@ typedef struct n2wos_bvh n2wos_bvh;
 /* Description of the last failure on the calling thread ("" if none). */
 const char* n2wos_last_error(void);
 int n2wos_abi_version(void);
-/* Launch tuning for benchmarks: key "ring" = LDS stack entries per lane of the
- * walker kernel (8 or 16; default 8).  Process-wide. */
+/* Launch tuning for benchmarks (process-wide): "ring" = LDS stack entries
+ * per lane of the walker (8 or 16, default 8); "refill_min" = idle lanes that
+ * trigger a refill (1..64, default 8); "weight_node" / "weight_tri" /
+ * "weight_update" = per-lane weights of the walker's phase choice. */
 int n2wos_set_tuning(const char* key, int value);
 /* Device name and runtime version of the current HIP device, for reports. */
 int n2wos_device_summary(char* buf, int buf_len);
@@ -181,6 +183,13 @@ typedef struct {
   uint64_t seed;      /* Philox key */
   uint32_t rng_stream;/* Philox counter word 3 = 2 rng_stream (+1) */
   int max_blocks;     /* 0 = fill the device */
+  /* Optional instrumentation (NULL = off): two uint64 device counters that
+   * the walker kernel adds its node-line loads and triangle tests to (a
+   * counting build of the same kernel, same results), and two hipEvent_t
+   * recorded on the stream right before and after the walker kernel. */
+  void* d_visit_counts;
+  void* event_begin;
+  void* event_end;
 } n2wos_walk_params;
 
 /* Bytes of device workspace n2wos_wos_solve needs for this job. */

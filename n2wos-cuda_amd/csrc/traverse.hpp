@@ -120,6 +120,11 @@ __device__ __forceinline__ Hit bvh4_closest(const Bvh4Node* __restrict__ nodes, 
           const float4* tp = reinterpret_cast<const float4*>(tris + first + i);
           const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
           if (COUNT) ++h.tris;
+          // the triangle's own box bounds its distance from below
+          const float bxl = fminf(fminf(t0.x, t0.w), t1.z), bxh = fmaxf(fmaxf(t0.x, t0.w), t1.z);
+          const float byl = fminf(fminf(t0.y, t1.x), t1.w), byh = fmaxf(fmaxf(t0.y, t1.x), t1.w);
+          const float bzl = fminf(fminf(t0.z, t1.y), t2.x), bzh = fmaxf(fmaxf(t0.z, t1.y), t2.x);
+          if (box_dist2(p, bxl, byl, bzl, bxh, byh, bzh) > thr) continue;
           const V3 a = v3(t0.x, t0.y, t0.z), b = v3(t0.w, t1.x, t1.y), c = v3(t1.z, t1.w, t2.x);
           const int id = __float_as_int(t2.y);
           const V3 q = closest_on_triangle(p, a, b, c);

@@ -1,0 +1,404 @@
+// walker_kernel.hpp -- device side of the persistent Walk-on-Spheres walker
+// (included by wos.hip; also compiled on the host by the SIMT emulator used in
+// the CPU tests, which supplies its own definitions of the wave intrinsics).
+#pragma once
+
+#include "traverse.hpp"
+#include "wos.hpp"
+
+namespace n2 {
+
+#ifndef N2_EMU_LOOP_HOOK
+#define N2_EMU_LOOP_HOOK()
+#define N2_EMU_TRACE(sN, sT, sU)
+#endif
+
+#ifndef N2_EMULATED_WAVE
+__device__ __forceinline__ int lane_id() {
+  return static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)));
+}
+
+// number of set bits of `mask` below this lane
+__device__ __forceinline__ int rank_in(uint64_t mask) {
+  return static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u)));
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+#endif
+
+// Chan et al. pairwise merge of (n, mean, M2); same expression order as the oracle.
+__host__ __device__ __forceinline__ void chan_merge(double na, double ma, double sa, double nb, double mb, double sb,
+                                                    double& n, double& m, double& s) {
+  if (nb == 0.0) {
+    n = na; m = ma; s = sa;
+    return;
+  }
+  if (na == 0.0) {
+    n = nb; m = mb; s = sb;
+    return;
+  }
+  const double nn = na + nb;
+  const double d = mb - ma;
+  n = nn;
+  m = ma + d * (nb / nn);
+  s = sa + sb + d * d * (na * nb / nn);
+}
+
+#ifndef N2_WAVES_PER_BLOCK
+#define N2_WAVES_PER_BLOCK 4
+#endif
+constexpr int kWaves = N2_WAVES_PER_BLOCK;  // waves per block
+constexpr int kBlock = 64 * kWaves;
+
+// Lane phases of the persistent walker.
+constexpr int kIdle = 0;    // no walk: waiting for refill
+constexpr int kNode = 1;    // query in progress, at an inner node
+constexpr int kTri = 2;     // query in progress, testing the triangles of a leaf
+constexpr int kUpdate = 3;  // query finished: take the walk step (or end the walk)
+
+// Every iteration the wave serves ONE phase, chosen by ballot (the phase with
+// the most lanes per instruction of its path), so divergent lanes never pay
+// for paths they do not take, and a lane whose query ends starts its next
+// query without waiting for the rest of the wave.  Each lane's own sequence of
+// node visits, triangle tests and walk steps is exactly the sequential one.
+template <int CAP, int S, bool POISSON, bool COUNT>
+__global__ void __launch_bounds__(kBlock) k_wos_persistent(WosKernelArgs a) {
+  __shared__ uint32_t s_ref[CAP * kBlock];
+  __shared__ float s_d2[CAP * kBlock];
+  __shared__ float s_val[kWaves][S][64];
+  __shared__ int s_unit[kWaves][S];
+  __shared__ int s_rem[kWaves][S];
+  __shared__ uint32_t s_steps[kWaves][S];
+  __shared__ uint32_t s_forced[kWaves][S];
+  __shared__ uint32_t s_ovf[kWaves][S];
+
+  const int lane = lane_id();
+  const int wv = static_cast<int>(threadIdx.x) >> 6;
+  uint32_t* sref = s_ref + threadIdx.x;
+  float* sd2 = s_d2 + threadIdx.x;
+  if (lane < S) s_unit[wv][lane] = -1;
+  wave_sync();
+
+  // wave-uniform fill cursor
+  int fill_slot = -1, fill_next = 0, fill_count = 0, fill_point = 0, fill_w0 = 0;
+  bool no_more = false;
+
+  // walk state
+  int phase = kIdle;
+  int slot = 0, wiu = 0, point = 0, walk = 0, step = 0;
+  V3 p = v3(0.0f, 0.0f, 0.0f);
+  float src = 0.0f;
+  float thr0 = FLT_MAX;
+  // query state
+  uint32_t cur = 0;
+  int ti = 0, te = 0, top = 0, cnt = 0, scnt = 0, bid = -1, ovf = 0;
+  float thr = FLT_MAX, bd2 = FLT_MAX;
+  V3 bcp = v3(0.0f, 0.0f, 0.0f);
+  uint32_t spill_ref[kSpill];
+  float spill_d2[kSpill];
+  uint32_t n_nodes = 0, n_tris = 0;
+
+  // go to the entry `r` of the traversal
+#define N2_ENTER(r)                                       \
+  do {                                                    \
+    const uint32_t enter_ref_ = (r);                      \
+    if (enter_ref_ & kLeafFlag) {                         \
+      ti = static_cast<int>((enter_ref_ >> 5) & 0x3ffffffu); \
+      te = ti + static_cast<int>(enter_ref_ & 31u);       \
+      phase = kTri;                                       \
+    } else {                                              \
+      cur = enter_ref_;                                   \
+      phase = kNode;                                      \
+    }                                                     \
+  } while (0)
+  // pop the next entry that can still hold a closer triangle, or finish
+#define N2_POP_NEXT()                                \
+  do {                                               \
+    bool pop_found_ = false;                         \
+    while (cnt > 0 || scnt > 0) {                    \
+      float pop_key_;                                \
+      uint32_t pop_ref_;                             \
+      if (cnt > 0) {                                 \
+        top = (top - 1) & (CAP - 1);                 \
+        --cnt;                                       \
+        pop_key_ = sd2[top * kBlock];                \
+        pop_ref_ = sref[top * kBlock];               \
+      } else {                                       \
+        --scnt;                                      \
+        pop_key_ = spill_d2[scnt];                   \
+        pop_ref_ = spill_ref[scnt];                  \
+      }                                              \
+      if (pop_key_ <= thr) {                         \
+        N2_ENTER(pop_ref_);                          \
+        pop_found_ = true;                           \
+        break;                                       \
+      }                                              \
+    }                                                \
+    if (!pop_found_) phase = kUpdate;                \
+  } while (0)
+#define N2_PUSH(R, K)                                \
+  do {                                               \
+    if (cnt == CAP) {                                \
+      if (scnt < kSpill) {                           \
+        spill_ref[scnt] = sref[top * kBlock];        \
+        spill_d2[scnt] = sd2[top * kBlock];          \
+        ++scnt;                                      \
+      } else {                                       \
+        ovf = 1;                                     \
+      }                                              \
+    } else {                                         \
+      ++cnt;                                         \
+    }                                                \
+    sref[top * kBlock] = (R);                        \
+    sd2[top * kBlock] = (K);                         \
+    top = (top + 1) & (CAP - 1);                     \
+  } while (0)
+
+  while (true) {
+    N2_EMU_LOOP_HOOK();
+    // ---------------------------------------------------------- refill
+    uint64_t idle = __ballot(phase == kIdle);
+    if (__popcll(idle) >= a.refill_min || __popcll(idle) == 64) {
+      while (idle != 0) {
+        if (fill_next == fill_count) {
+          if (no_more) break;
+          int fs = -1;
+          for (int s = 0; s < S; ++s) {
+            if (s_unit[wv][s] < 0) {
+              fs = s;
+              break;
+            }
+          }
+          if (fs < 0) break;
+          int u = 0;
+          if (lane == 0) u = atomicAdd(a.next_unit, 1);
+          u = __shfl(u, 0);
+          if (u >= a.total_units) {
+            no_more = true;
+            break;
+          }
+          const int pt = u / a.units_per_point;
+          const int chunk = u - pt * a.units_per_point;
+          const int ucnt = min(64, a.walks_per_point - 64 * chunk);
+          if (lane == 0) {
+            s_unit[wv][fs] = u;
+            s_rem[wv][fs] = ucnt;
+            s_steps[wv][fs] = 0u;
+            s_forced[wv][fs] = 0u;
+            s_ovf[wv][fs] = 0u;
+          }
+          wave_sync();
+          fill_slot = fs;
+          fill_next = 0;
+          fill_count = ucnt;
+          fill_point = pt;
+          fill_w0 = 64 * chunk;
+        }
+        const int take = min(__popcll(idle), fill_count - fill_next);
+        const bool is_idle = (idle >> lane) & 1ull;
+        const int r = rank_in(idle);
+        if (is_idle && r < take) {
+          // every walk of a point shares its first query: computed once per point
+          slot = fill_slot;
+          wiu = fill_next + r;
+          point = fill_point;
+          walk = fill_w0 + wiu;
+          step = 0;
+          src = 0.0f;
+          thr0 = FLT_MAX;
+          p = v3(a.points[3 * point], a.points[3 * point + 1], a.points[3 * point + 2]);
+          bd2 = a.first_d2[point];
+          bcp = v3(a.first_cp[3 * point], a.first_cp[3 * point + 1], a.first_cp[3 * point + 2]);
+          bid = 0;
+          ovf = a.first_ovf[point];
+          phase = kUpdate;
+        }
+        fill_next += take;
+        idle = __ballot(phase == kIdle);
+      }
+    }
+
+    const uint64_t mN = __ballot(phase == kNode);
+    const uint64_t mT = __ballot(phase == kTri);
+    const uint64_t mU = __ballot(phase == kUpdate);
+    if ((mN | mT | mU) == 0) {
+      if (no_more || __popcll(idle) < 64) break;
+      continue;
+    }
+    const int sN = __popcll(mN) * a.weight_node;
+    const int sT = __popcll(mT) * a.weight_tri;
+    const int sU = __popcll(mU) * a.weight_update;
+    N2_EMU_TRACE(sN, sT, sU);
+
+    if (sU >= sN && sU >= sT) {
+      // ------------------------------------------------------ walk step
+      bool unit_done = false;
+      if (phase == kUpdate) {
+        if (bid < 0 && thr0 < FLT_MAX && finite3(p)) {
+          // the step bound was too tight (cannot happen in exact arithmetic): redo unbounded
+          thr0 = FLT_MAX;
+          thr = FLT_MAX;
+          top = cnt = scnt = 0;
+          cur = 0;
+          phase = kNode;
+        } else {
+          const float d = sqrtf(fmaxf(bd2, 0.0f));
+          const bool forced = step >= a.max_steps;
+          if (d <= a.epsilon || forced || !isfinite(d)) {
+            const float value = src + boundary_value(a.bc, bcp);
+            s_val[wv][slot][wiu] = value;
+            atomicAdd(&s_steps[wv][slot], static_cast<uint32_t>(step));
+            if (forced) atomicAdd(&s_forced[wv][slot], 1u);
+            if (ovf) atomicAdd(&s_ovf[wv][slot], 1u);
+            if (a.walk_values) {
+              const int64_t sidx = static_cast<int64_t>(point) * a.walks_per_point + walk;
+              a.walk_values[sidx] = value;
+              a.walk_steps[sidx] = step;
+            }
+            const int old = atomicSub(&s_rem[wv][slot], 1);
+            unit_done = (old == 1);
+            phase = kIdle;
+          } else {
+            const uint32_t gp = static_cast<uint32_t>(point) + a.point_offset;
+            const U4 rr = philox4x32_10(
+                U4{static_cast<uint32_t>(step), static_cast<uint32_t>(walk), gp, 2u * a.stream}, a.seed_lo,
+                a.seed_hi);
+            const V3 dir = sphere_dir(rr.x, rr.y);
+            const float R = a.step_scale * d;
+            if (POISSON) {
+              const U4 t = philox4x32_10(
+                  U4{static_cast<uint32_t>(step), static_cast<uint32_t>(walk), gp, 2u * a.stream + 1u}, a.seed_lo,
+                  a.seed_hi);
+              const float rho = median3(u01(rr.z), u01(rr.w), u01(t.x));
+              const V3 e = sphere_dir(t.y, t.z);
+              const float Rr = R * rho;
+              const V3 y = v3(p.x + Rr * e.x, p.y + Rr * e.y, p.z + Rr * e.z);
+              src = src + (R * R) * poisson_source(y) / 6.0f;
+            }
+            p = v3(p.x + R * dir.x, p.y + R * dir.y, p.z + R * dir.z);
+            ++step;
+            // |p_new - closest| <= d + |R dir|: bound the next query (loose
+            // margin covers FP32 rounding of d, R, dir and p)
+            const float tb = (d + R) * 1.0000152587890625f + 4.0f * a.delta;
+            thr0 = tb * tb;
+            bd2 = FLT_MAX;
+            bcp = v3(0.0f, 0.0f, 0.0f);
+            bid = -1;
+            ovf = 0;
+            if (finite3(p)) {
+              thr = thr0;
+              top = cnt = scnt = 0;
+              cur = 0;
+              phase = kNode;
+            } else {
+              bd2 = INFINITY;  // non-finite position: the walk ends at the next update
+            }
+          }
+        }
+      }
+      wave_sync();
+      // ---------------------------------------------------- finish units
+      uint64_t done = __ballot(unit_done);
+      while (done != 0) {
+        const int l = __ffsll(static_cast<unsigned long long>(done)) - 1;
+        done &= done - 1;
+        const int s = __shfl(slot, l);
+        const int u = s_unit[wv][s];
+        const int pt = u / a.units_per_point;
+        const int ucnt = min(64, a.walks_per_point - 64 * (u - pt * a.units_per_point));
+        double n = lane < ucnt ? 1.0 : 0.0;
+        double m = lane < ucnt ? static_cast<double>(s_val[wv][s][lane]) : 0.0;
+        double q = 0.0;
+#pragma unroll
+        for (int stride = 1; stride < 64; stride *= 2) {
+          const double nb = __shfl_down(n, stride);
+          const double mb = __shfl_down(m, stride);
+          const double qb = __shfl_down(q, stride);
+          if ((lane & (2 * stride - 1)) == 0) chan_merge(n, m, q, nb, mb, qb, n, m, q);
+        }
+        if (lane == 0) {
+          a.unit_mean[u] = m;
+          a.unit_m2[u] = q;
+          a.unit_steps[u] = s_steps[wv][s];
+          a.unit_flags[u] = (s_forced[wv][s] & 0xffffu) | (s_ovf[wv][s] << 16);
+          s_unit[wv][s] = -1;
+        }
+        wave_sync();
+      }
+    } else if (sN >= sT) {
+      // ------------------------------------------------------ one node visit
+      if (phase == kNode) {
+        const float4* np = reinterpret_cast<const float4*>(a.nodes + cur);
+        const float4 lox = np[0], loy = np[1], loz = np[2];
+        const float4 hix = np[3], hiy = np[4], hiz = np[5];
+        const uint4 ch = reinterpret_cast<const uint4*>(np)[6];
+        if (COUNT) ++n_nodes;
+        float k0 = box_dist2(p, lox.x, loy.x, loz.x, hix.x, hiy.x, hiz.x);
+        float k1 = box_dist2(p, lox.y, loy.y, loz.y, hix.y, hiy.y, hiz.y);
+        float k2 = box_dist2(p, lox.z, loy.z, loz.z, hix.z, hiy.z, hiz.z);
+        float k3 = box_dist2(p, lox.w, loy.w, loz.w, hix.w, hiy.w, hiz.w);
+        const int nvalid = (k0 <= thr) + (k1 <= thr) + (k2 <= thr) + (k3 <= thr);
+        if (nvalid == 0) {
+          N2_POP_NEXT();
+        } else {
+          uint32_t r0 = ch.x, r1 = ch.y, r2 = ch.z, r3 = ch.w;
+          int s0 = 0, s1 = 1, s2 = 2, s3 = 3;
+          k0 = k0 <= thr ? k0 : INFINITY;
+          k1 = k1 <= thr ? k1 : INFINITY;
+          k2 = k2 <= thr ? k2 : INFINITY;
+          k3 = k3 <= thr ? k3 : INFINITY;
+          cswap(k0, r0, s0, k1, r1, s1);
+          cswap(k2, r2, s2, k3, r3, s3);
+          cswap(k0, r0, s0, k2, r2, s2);
+          cswap(k1, r1, s1, k3, r3, s3);
+          cswap(k1, r1, s1, k2, r2, s2);
+          if (nvalid >= 4) N2_PUSH(r3, k3);
+          if (nvalid >= 3) N2_PUSH(r2, k2);
+          if (nvalid >= 2) N2_PUSH(r1, k1);
+          N2_ENTER(r0);
+        }
+      }
+    } else {
+      // ------------------------------------------------------ one triangle
+      if (phase == kTri) {
+        const float4* tp = reinterpret_cast<const float4*>(a.tris + ti);
+        const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+        if (COUNT) ++n_tris;
+        // the triangle's own box bounds its distance from below: skip the
+        // exact test when even the box is beyond the pruning threshold
+        const float bxl = fminf(fminf(t0.x, t0.w), t1.z), bxh = fmaxf(fmaxf(t0.x, t0.w), t1.z);
+        const float byl = fminf(fminf(t0.y, t1.x), t1.w), byh = fmaxf(fmaxf(t0.y, t1.x), t1.w);
+        const float bzl = fminf(fminf(t0.z, t1.y), t2.x), bzh = fmaxf(fmaxf(t0.z, t1.y), t2.x);
+        if (box_dist2(p, bxl, byl, bzl, bxh, byh, bzh) <= thr) {
+          const V3 ta = v3(t0.x, t0.y, t0.z), tb = v3(t0.w, t1.x, t1.y), tc = v3(t1.z, t1.w, t2.x);
+          const int id = __float_as_int(t2.y);
+          const V3 q = closest_on_triangle(p, ta, tb, tc);
+          const V3 e = q - p;
+          const float d2 = dot(e, e);
+          if (d2 < bd2 || (d2 == bd2 && id < bid)) {
+            bd2 = d2;
+            bcp = q;
+            bid = id;
+            thr = fminf(thr, prune_threshold(d2, a.delta));
+          }
+        }
+        ++ti;
+        if (ti == te) N2_POP_NEXT();
+      }
+    }
+  }
+#undef N2_ENTER
+#undef N2_POP_NEXT
+#undef N2_PUSH
+  if (COUNT) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(a.visits), static_cast<unsigned long long>(n_nodes));
+    atomicAdd(reinterpret_cast<unsigned long long*>(a.visits) + 1, static_cast<unsigned long long>(n_tris));
+  }
+}
+
+
+}  // namespace n2

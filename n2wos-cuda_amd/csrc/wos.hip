@@ -39,226 +39,11 @@
 #include <algorithm>
 
 #include "api_internal.hpp"
-#include "traverse.hpp"
-#include "wos.hpp"
+#include "walker_kernel.hpp"
 
 namespace n2 {
 
 namespace {
-
-__device__ __forceinline__ int lane_id() {
-  return static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)));
-}
-
-// number of set bits of `mask` below this lane
-__device__ __forceinline__ int rank_in(uint64_t mask) {
-  return static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
-                                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u)));
-}
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-
-// Chan et al. pairwise merge of (n, mean, M2); same expression order as the oracle.
-__host__ __device__ __forceinline__ void chan_merge(double na, double ma, double sa, double nb, double mb, double sb,
-                                                    double& n, double& m, double& s) {
-  if (nb == 0.0) {
-    n = na; m = ma; s = sa;
-    return;
-  }
-  if (na == 0.0) {
-    n = nb; m = mb; s = sb;
-    return;
-  }
-  const double nn = na + nb;
-  const double d = mb - ma;
-  n = nn;
-  m = ma + d * (nb / nn);
-  s = sa + sb + d * d * (na * nb / nn);
-}
-
-constexpr int kWaves = 4;        // waves per block
-constexpr int kBlock = 64 * kWaves;
-
-template <int CAP, int S, bool POISSON>
-__global__ void __launch_bounds__(kBlock) k_wos_persistent(WosKernelArgs a) {
-  __shared__ uint32_t s_ref[CAP * kBlock];
-  __shared__ float s_d2[CAP * kBlock];
-  __shared__ float s_val[kWaves][S][64];
-  __shared__ int s_unit[kWaves][S];
-  __shared__ int s_rem[kWaves][S];
-  __shared__ uint32_t s_steps[kWaves][S];
-  __shared__ uint32_t s_forced[kWaves][S];
-  __shared__ uint32_t s_ovf[kWaves][S];
-
-  const int lane = lane_id();
-  const int wv = static_cast<int>(threadIdx.x) >> 6;
-  uint32_t* sref = s_ref + threadIdx.x;
-  float* sd2 = s_d2 + threadIdx.x;
-  if (lane < S) s_unit[wv][lane] = -1;
-  wave_sync();
-
-  // wave-uniform fill cursor
-  int fill_slot = -1, fill_next = 0, fill_count = 0, fill_point = 0, fill_w0 = 0, fill_unit = 0;
-  bool no_more = false;
-
-  // lane state
-  bool active = false;
-  int slot = 0, wiu = 0, point = 0, walk = 0, step = 0;
-  V3 p = v3(0.0f, 0.0f, 0.0f);
-  float src = 0.0f;
-  float thr0 = FLT_MAX;
-
-  while (true) {
-    // ---------------------------------------------------------- refill
-    uint64_t idle = __ballot(!active);
-    while (idle != 0) {
-      if (fill_next == fill_count) {
-        if (no_more) break;
-        int fs = -1;
-        for (int s = 0; s < S; ++s) {
-          if (s_unit[wv][s] < 0) {
-            fs = s;
-            break;
-          }
-        }
-        if (fs < 0) break;
-        int u = 0;
-        if (lane == 0) u = atomicAdd(a.next_unit, 1);
-        u = __shfl(u, 0);
-        if (u >= a.total_units) {
-          no_more = true;
-          break;
-        }
-        const int pt = u / a.units_per_point;
-        const int chunk = u - pt * a.units_per_point;
-        const int cnt = min(64, a.walks_per_point - 64 * chunk);
-        if (lane == 0) {
-          s_unit[wv][fs] = u;
-          s_rem[wv][fs] = cnt;
-          s_steps[wv][fs] = 0u;
-          s_forced[wv][fs] = 0u;
-          s_ovf[wv][fs] = 0u;
-        }
-        wave_sync();
-        fill_slot = fs;
-        fill_next = 0;
-        fill_count = cnt;
-        fill_point = pt;
-        fill_w0 = 64 * chunk;
-        fill_unit = u;
-      }
-      const int take = min(__popcll(idle), fill_count - fill_next);
-      const bool is_idle = (idle >> lane) & 1ull;
-      const int r = rank_in(idle);
-      if (is_idle && r < take) {
-        active = true;
-        slot = fill_slot;
-        wiu = fill_next + r;
-        point = fill_point;
-        walk = fill_w0 + wiu;
-        step = 0;
-        src = 0.0f;
-        thr0 = FLT_MAX;
-        p = v3(a.points[3 * point], a.points[3 * point + 1], a.points[3 * point + 2]);
-      }
-      fill_next += take;
-      idle = __ballot(!active);
-    }
-    (void)fill_unit;
-    if (__ballot(active) == 0) break;
-
-    // ---------------------------------------------------------- one step
-    bool unit_done = false;
-    if (active) {
-      Hit h;
-      if (step == 0) {
-        // every walk of a point shares its first query: computed once per point
-        h.d2 = a.first_d2[point];
-        h.cp = v3(a.first_cp[3 * point], a.first_cp[3 * point + 1], a.first_cp[3 * point + 2]);
-        h.id = 0;
-        h.overflow = a.first_ovf[point];
-      } else if (finite3(p)) {
-        h = bvh4_closest<CAP, false>(a.nodes, a.tris, p, a.delta, thr0, sref, sd2, kBlock);
-        if (h.id < 0 && thr0 < FLT_MAX)  // bound too tight (cannot happen in exact arithmetic)
-          h = bvh4_closest<CAP, false>(a.nodes, a.tris, p, a.delta, FLT_MAX, sref, sd2, kBlock);
-      } else {
-        h.d2 = FLT_MAX; h.cp = v3(0.0f, 0.0f, 0.0f); h.id = -1; h.overflow = 0;
-      }
-      const float d = sqrtf(fmaxf(h.d2, 0.0f));
-      const bool forced = step >= a.max_steps;
-      if (d <= a.epsilon || forced || !isfinite(d)) {
-        const float value = src + boundary_value(a.bc, h.cp);
-        s_val[wv][slot][wiu] = value;
-        atomicAdd(&s_steps[wv][slot], static_cast<uint32_t>(step));
-        if (forced) atomicAdd(&s_forced[wv][slot], 1u);
-        if (h.overflow) atomicAdd(&s_ovf[wv][slot], 1u);
-        if (a.walk_values) {
-          const int64_t sidx = static_cast<int64_t>(point) * a.walks_per_point + walk;
-          a.walk_values[sidx] = value;
-          a.walk_steps[sidx] = step;
-        }
-        const int old = atomicSub(&s_rem[wv][slot], 1);
-        unit_done = (old == 1);
-        active = false;
-      } else {
-        const uint32_t gp = static_cast<uint32_t>(point) + a.point_offset;
-        const U4 rr = philox4x32_10(U4{static_cast<uint32_t>(step), static_cast<uint32_t>(walk), gp, 2u * a.stream},
-                                    a.seed_lo, a.seed_hi);
-        const V3 dir = sphere_dir(rr.x, rr.y);
-        const float R = a.step_scale * d;
-        if (POISSON) {
-          const U4 t = philox4x32_10(
-              U4{static_cast<uint32_t>(step), static_cast<uint32_t>(walk), gp, 2u * a.stream + 1u}, a.seed_lo,
-              a.seed_hi);
-          const float rho = median3(u01(rr.z), u01(rr.w), u01(t.x));
-          const V3 e = sphere_dir(t.y, t.z);
-          const float Rr = R * rho;
-          const V3 y = v3(p.x + Rr * e.x, p.y + Rr * e.y, p.z + Rr * e.z);
-          src = src + (R * R) * poisson_source(y) / 6.0f;
-        }
-        p = v3(p.x + R * dir.x, p.y + R * dir.y, p.z + R * dir.z);
-        ++step;
-        // |p_new - closest| <= d + |R dir|: bound the next query (loose margin
-        // covers FP32 rounding of d, R, dir and p)
-        const float t = (d + R) * 1.0000152587890625f + 4.0f * a.delta;
-        thr0 = t * t;
-      }
-    }
-    wave_sync();
-
-    // ---------------------------------------------------------- finish units
-    uint64_t done = __ballot(unit_done);
-    while (done != 0) {
-      const int l = __ffsll(static_cast<unsigned long long>(done)) - 1;
-      done &= done - 1;
-      const int s = __shfl(slot, l);
-      const int u = s_unit[wv][s];
-      const int pt = u / a.units_per_point;
-      const int cnt = min(64, a.walks_per_point - 64 * (u - pt * a.units_per_point));
-      double n = lane < cnt ? 1.0 : 0.0;
-      double m = lane < cnt ? static_cast<double>(s_val[wv][s][lane]) : 0.0;
-      double q = 0.0;
-#pragma unroll
-      for (int stride = 1; stride < 64; stride *= 2) {
-        const double nb = __shfl_down(n, stride);
-        const double mb = __shfl_down(m, stride);
-        const double qb = __shfl_down(q, stride);
-        if ((lane & (2 * stride - 1)) == 0) chan_merge(n, m, q, nb, mb, qb, n, m, q);
-      }
-      if (lane == 0) {
-        a.unit_mean[u] = m;
-        a.unit_m2[u] = q;
-        a.unit_steps[u] = s_steps[wv][s];
-        a.unit_flags[u] = (s_forced[wv][s] & 0xffffu) | (s_ovf[wv][s] << 16);
-        s_unit[wv][s] = -1;
-      }
-      wave_sync();
-    }
-  }
-}
 
 // Merge each point's units left to right into its (n, mean, M2) state.
 __global__ void k_point_merge(int n_points, int units_per_point, int walks_per_point, int accumulate,
@@ -295,13 +80,17 @@ __global__ void k_zero_counter(int* c) { *c = 0; }
 
 template <int CAP, int S, bool POISSON>
 void launch_wos(const WosKernelArgs& a, int grid, hipStream_t stream) {
-  hipLaunchKernelGGL((k_wos_persistent<CAP, S, POISSON>), dim3(grid), dim3(kBlock), 0, stream, a);
+  if (a.visits)
+    hipLaunchKernelGGL((k_wos_persistent<CAP, S, POISSON, true>), dim3(grid), dim3(kBlock), 0, stream, a);
+  else
+    hipLaunchKernelGGL((k_wos_persistent<CAP, S, POISSON, false>), dim3(grid), dim3(kBlock), 0, stream, a);
 }
 
 template <int CAP, int S, bool POISSON>
 int max_blocks_per_cu() {
   int b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wos_persistent<CAP, S, POISSON>, kBlock, 0) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wos_persistent<CAP, S, POISSON, false>, kBlock, 0) !=
+      hipSuccess)
     return 1;
   return std::max(1, b);
 }
@@ -368,11 +157,18 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
   if (r.max_blocks > 0) grid = std::min(grid, r.max_blocks);
   const int64_t needed = (units + kWaves - 1) / kWaves;
   grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(grid, needed)));
+  a.visits = r.d_visits;
+  a.refill_min = tuning_value("refill_min", 8);
+  a.weight_node = tuning_value("weight_node", 10);
+  a.weight_tri = tuning_value("weight_tri", 10);
+  a.weight_update = tuning_value("weight_update", 6);
+  if (r.ev_begin) N2_HIP(hipEventRecord(static_cast<hipEvent_t>(r.ev_begin), stream));
   if (cap == 16) {
     if (poisson) launch_wos<16, kSlots, true>(a, grid, stream); else launch_wos<16, kSlots, false>(a, grid, stream);
   } else {
     if (poisson) launch_wos<8, kSlots, true>(a, grid, stream); else launch_wos<8, kSlots, false>(a, grid, stream);
   }
+  if (r.ev_end) N2_HIP(hipEventRecord(static_cast<hipEvent_t>(r.ev_end), stream));
   N2_HIP(hipGetLastError());
   const int mb = 256;
   hipLaunchKernelGGL(k_point_merge, dim3((r.n_points + mb - 1) / mb), dim3(mb), 0, stream, r.n_points,
@@ -449,6 +245,9 @@ extern "C" int n2wos_wos_solve(const n2wos_bvh* bvh, const n2wos_boundary* bound
     r.d_workspace = d_workspace;
     r.d_walk_values = d_walk_values;
     r.d_walk_steps = d_walk_steps;
+    r.d_visits = static_cast<uint64_t*>(wp->d_visit_counts);
+    r.ev_begin = wp->event_begin;
+    r.ev_end = wp->event_end;
     run_wos(bvh, bc, r, static_cast<hipStream_t>(stream));
     return N2WOS_OK;
   });

@@ -36,6 +36,11 @@ struct WosKernelArgs {
   const float* first_d2;  // first query of each point (shared by its walks)
   const float* first_cp;
   const int* first_ovf;
+  uint64_t* visits;       // optional: [node lines loaded, triangle records tested]
+  int refill_min;         // refill when at least this many lanes are idle
+  int weight_node;        // phase choice: lanes x weight, largest wins
+  int weight_tri;
+  int weight_update;
 };
 
 struct WosRun {
@@ -59,6 +64,9 @@ struct WosRun {
   void* d_workspace;
   float* d_walk_values;
   int* d_walk_steps;
+  uint64_t* d_visits;
+  void* ev_begin;
+  void* ev_end;
 };
 
 int64_t wos_workspace_bytes(int n_points, int walks_per_point);

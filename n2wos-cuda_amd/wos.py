@@ -26,7 +26,8 @@ class BoundaryC(C.Structure):
 
 class WalkParamsC(C.Structure):
     _fields_ = [("max_steps", C.c_int), ("epsilon", C.c_float), ("step_scale", C.c_float), ("source", C.c_int),
-                ("seed", C.c_uint64), ("rng_stream", C.c_uint32), ("max_blocks", C.c_int)]
+                ("seed", C.c_uint64), ("rng_stream", C.c_uint32), ("max_blocks", C.c_int),
+                ("d_visit_counts", C.c_void_p), ("event_begin", C.c_void_p), ("event_end", C.c_void_p)]
 
 
 @dataclass
@@ -65,7 +66,7 @@ class WalkParams:
 
     def to_c(self) -> WalkParamsC:
         return WalkParamsC(self.max_steps, self.epsilon, self.step_scale, self.source, self.seed, self.rng_stream,
-                           self.max_blocks)
+                           self.max_blocks, None, None, None)
 
 
 @dataclass
@@ -119,7 +120,8 @@ class Solver:
         return self._ws
 
     def solve(self, points, walks_per_point: int, stats: PointStats | None = None, accumulate: bool = False,
-              point_offset: int = 0, per_walk: bool = False, stream=None, seed: int | None = None):
+              point_offset: int = 0, per_walk: bool = False, stream=None, seed: int | None = None,
+              visit_counts=None, events=None):
         """Runs walks_per_point walks from each point of a (n, 3) CUDA float32 tensor."""
         import torch
         n = points.shape[0]
@@ -131,9 +133,14 @@ class Solver:
             wv = torch.empty(n * walks_per_point, dtype=torch.float32, device=points.device)
             ws_steps = torch.empty(n * walks_per_point, dtype=torch.int32, device=points.device)
         wp = self._wp
-        if seed is not None:
+        if seed is not None or visit_counts is not None or events is not None:
             wp = self.params.to_c()
-            wp.seed = seed
+            if seed is not None:
+                wp.seed = seed
+            if visit_counts is not None:
+                wp.d_visit_counts = visit_counts.data_ptr()
+            if events is not None:
+                wp.event_begin, wp.event_end = events
         s = torch.cuda.current_stream().cuda_stream if stream is None else getattr(stream, "cuda_stream", stream)
         check(lib().n2wos_wos_solve(self.bvh.handle, C.byref(self._bc), C.byref(wp), points.data_ptr(), n,
                                     walks_per_point, point_offset, stats.n.data_ptr(), stats.mean.data_ptr(),

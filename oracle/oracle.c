@@ -751,13 +751,22 @@ static inline v3 dir_from(uint32_t u0, uint32_t u1) {
   return mk(r * c, r * s, z);
 }
 
-/* returns 0 boundary, 1 forced (max steps), 2 stopped at depth */
+/* returns 0 boundary, 1 forced (max steps), 2 stopped at depth.  first_*
+ * optionally supplies step 0's query (identical for every walk of a point). */
 static int walk_one(const orc_bvh2* b, const orc_walk_opts* o, v3 p, uint32_t point, uint32_t walk, int depth,
-                    float* value, int* steps, v3* xm) {
+                    const float* first_d2, const v3* first_cp, float* value, int* steps, v3* xm) {
   float src = 0.0f;
   for (int step = 0; step <= o->max_steps; ++step) {
     float d2; v3 P; int tri;
-    bvh2_query(b, p, &d2, &P, &tri);
+    if (step == 0 && first_d2) {
+      d2 = *first_d2;
+      P = *first_cp;
+    } else if (!finite3(p)) {
+      d2 = INFINITY; /* a non-finite position ends the walk */
+      P = mk(0.0f, 0.0f, 0.0f);
+    } else {
+      bvh2_query(b, p, &d2, &P, &tri);
+    }
     const float d = sqrtf(fmaxf(d2, 0.0f));
     const int forced = step >= o->max_steps;
     if (d <= o->eps || forced || !isfinite(d)) {
@@ -796,30 +805,58 @@ static int walk_one(const orc_bvh2* b, const orc_walk_opts* o, v3 p, uint32_t po
   return 1;
 }
 
+/* Shared state of one multi-threaded walk job: samples are handed out in
+ * chunks from an atomic counter (dynamic scheduling), after a first pass that
+ * computes each point's step-0 query once. */
 typedef struct {
   const orc_bvh2* b;
   const orc_walk_opts* o;
   const float* pts;
   const uint32_t* point_ids;
+  long npts;
   int wpp;
   int depth;
   float* values;
   int* steps;
   int* status;
   float* xm;
-  long begin, end;
-} wk_job;
+  float* first_d2;
+  v3* first_cp;
+  long total;
+  long next_point;
+  long next_sample;
+} wk_shared;
 
 static void* wk_worker(void* arg) {
-  wk_job* j = (wk_job*)arg;
-  for (long s = j->begin; s < j->end; ++s) {
-    const long pi = s / j->wpp;
-    const uint32_t w = (uint32_t)(s - pi * j->wpp);
-    const uint32_t pid = j->point_ids ? j->point_ids[pi] : (uint32_t)pi;
-    v3 xm;
-    const int st = walk_one(j->b, j->o, vld(j->pts + 3 * pi), pid, w, j->depth, &j->values[s], &j->steps[s], &xm);
-    if (j->status) j->status[s] = st;
-    if (j->xm) vst(j->xm + 3 * s, xm);
+  wk_shared* j = *(wk_shared**)arg;
+  for (;;) {
+    const long p0 = __atomic_fetch_add(&j->next_point, 64, __ATOMIC_RELAXED);
+    if (p0 >= j->npts) break;
+    const long p1 = p0 + 64 < j->npts ? p0 + 64 : j->npts;
+    for (long pi = p0; pi < p1; ++pi) {
+      int tri;
+      bvh2_query(j->b, vld(j->pts + 3 * pi), &j->first_d2[pi], &j->first_cp[pi], &tri);
+    }
+  }
+  return NULL;
+}
+
+static void* wk_worker2(void* arg) {
+  wk_shared* j = *(wk_shared**)arg;
+  for (;;) {
+    const long s0 = __atomic_fetch_add(&j->next_sample, 16, __ATOMIC_RELAXED);
+    if (s0 >= j->total) break;
+    const long s1 = s0 + 16 < j->total ? s0 + 16 : j->total;
+    for (long s = s0; s < s1; ++s) {
+      const long pi = s / j->wpp;
+      const uint32_t w = (uint32_t)(s - pi * j->wpp);
+      const uint32_t pid = j->point_ids ? j->point_ids[pi] : (uint32_t)pi;
+      v3 xm;
+      const int st = walk_one(j->b, j->o, vld(j->pts + 3 * pi), pid, w, j->depth, &j->first_d2[pi],
+                              &j->first_cp[pi], &j->values[s], &j->steps[s], &xm);
+      if (j->status) j->status[s] = st;
+      if (j->xm) vst(j->xm + 3 * s, xm);
+    }
   }
   return NULL;
 }
@@ -833,14 +870,20 @@ int orc_walks(const orc_bvh2* b, int max_steps, float eps, float step_scale, int
   orc_walk_opts o = {max_steps, eps, step_scale, bmode, bparams, poisson, (uint32_t)seed, (uint32_t)(seed >> 32), stream};
   if (nthreads < 1) nthreads = 1;
   if (nthreads > 256) nthreads = 256;
-  wk_job jobs[256];
-  const long total = npts * wpp;
-  for (int t = 0; t < nthreads; ++t) {
-    jobs[t].b = b; jobs[t].o = &o; jobs[t].pts = pts; jobs[t].point_ids = point_ids; jobs[t].wpp = wpp;
-    jobs[t].depth = depth; jobs[t].values = values; jobs[t].steps = steps; jobs[t].status = status; jobs[t].xm = xm;
-    jobs[t].begin = total * t / nthreads; jobs[t].end = total * (t + 1) / nthreads;
-  }
-  run_threads(wk_worker, jobs, sizeof(wk_job), nthreads);
+  wk_shared sh;
+  sh.b = b; sh.o = &o; sh.pts = pts; sh.point_ids = point_ids; sh.npts = npts; sh.wpp = wpp; sh.depth = depth;
+  sh.values = values; sh.steps = steps; sh.status = status; sh.xm = xm;
+  sh.first_d2 = (float*)malloc(sizeof(float) * (size_t)(npts > 0 ? npts : 1));
+  sh.first_cp = (v3*)malloc(sizeof(v3) * (size_t)(npts > 0 ? npts : 1));
+  sh.total = npts * wpp;
+  sh.next_point = 0;
+  sh.next_sample = 0;
+  wk_shared* ptrs[256];
+  for (int t = 0; t < nthreads; ++t) ptrs[t] = &sh;
+  run_threads(wk_worker, ptrs, sizeof(wk_shared*), nthreads);
+  run_threads(wk_worker2, ptrs, sizeof(wk_shared*), nthreads);
+  free(sh.first_d2);
+  free(sh.first_cp);
   return 0;
 }
 
