@@ -8,6 +8,7 @@ Workload (one "step" = one full solve of this batch, inputs resident in HBM):
   [-1, 1]^2 at z = 0, 256 walks per point, eps = 1e-4, step scale 0.999, at
   most 256 steps, Philox4x32-10 seeded per (point, walk, step), 8-charge
   harmonic Dirichlet data, per-point FP64 mean/variance on the device.
+  The BVH is built with leaves of at most 4 triangles.
 
 Prints ONE JSON line.  `value` is walk-steps/s of the whole job.  `roofline`
 prices the walker kernel's algorithmic bytes (node lines x 112 B + triangle
@@ -44,6 +45,9 @@ def parse():
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run a 1 GiB device copy (counter calibration) and one walker solve, then exit; "
+                         "used under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
 
 
@@ -70,7 +74,7 @@ def main():
 
     torch.cuda.set_device(0)
     verts, tris = P.make_mesh("mesh2")
-    bvh = eng.Bvh(verts, tris)
+    bvh = eng.Bvh(verts, tris, leaf_size=4)
     pts, _ = P.grid_slice_points(bvh, res=args.grid)
     n = len(pts)
     bc = P.charges_boundary(0)
@@ -79,6 +83,17 @@ def main():
     tp = torch.from_numpy(pts).cuda()
     stats = [W.PointStats.zeros(n) for _ in range(args.steps)]
     events = [hiptime.EventPair() for _ in range(args.steps)]
+
+    if args.traffic_probe:
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        solver.solve(tp, args.wpp, seed=params.seed)
+        torch.cuda.synchronize()
+        print(json.dumps({"traffic_probe": "done", "copy_bytes": 1 << 30, "points": n}))
+        return
 
     for w in range(args.warmup):
         solver.solve(tp, args.wpp, seed=1000 + w)

@@ -49,7 +49,7 @@ __host__ __device__ __forceinline__ void chan_merge(double na, double ma, double
 }
 
 #ifndef N2_WAVES_PER_BLOCK
-#define N2_WAVES_PER_BLOCK 4
+#define N2_WAVES_PER_BLOCK 1
 #endif
 constexpr int kWaves = N2_WAVES_PER_BLOCK;  // waves per block
 constexpr int kBlock = 64 * kWaves;

@@ -147,7 +147,7 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
   N2_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   constexpr int kSlots = 8;
   const bool poisson = r.source == N2WOS_SOURCE_POISSON_SINES;
-  const int cap = tuning_value("ring", 8);
+  const int cap = tuning_value("ring", 16);
   int per_cu;
   if (cap == 16)
     per_cu = poisson ? max_blocks_per_cu<16, kSlots, true>() : max_blocks_per_cu<16, kSlots, false>();
@@ -158,10 +158,10 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
   const int64_t needed = (units + kWaves - 1) / kWaves;
   grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(grid, needed)));
   a.visits = r.d_visits;
-  a.refill_min = tuning_value("refill_min", 8);
+  a.refill_min = tuning_value("refill_min", 2);
   a.weight_node = tuning_value("weight_node", 10);
   a.weight_tri = tuning_value("weight_tri", 10);
-  a.weight_update = tuning_value("weight_update", 6);
+  a.weight_update = tuning_value("weight_update", 10);
   if (r.ev_begin) N2_HIP(hipEventRecord(static_cast<hipEvent_t>(r.ev_begin), stream));
   if (cap == 16) {
     if (poisson) launch_wos<16, kSlots, true>(a, grid, stream); else launch_wos<16, kSlots, false>(a, grid, stream);

@@ -971,9 +971,14 @@ static inline void cswap4(float* ka, uint32_t* ra, int* sa, float* kb, uint32_t*
   }
 }
 
+static int g_bvh4_maxdepth; /* deepest stack of the last query (diagnostics) */
+static int g_bvh4_pushes;   /* pushes of the last query */
+
 static void bvh4_query(const orc_node4* nodes, const orc_tri4* tris, float delta, v3 p, float thr_init, int cap,
                        float* out_d2, v3* out_cp, int* out_id, int* out_ovf, uint32_t* out_nodes,
                        uint32_t* out_tris) {
+  g_bvh4_maxdepth = 0;
+  g_bvh4_pushes = 0;
   float best = FLT_MAX;
   v3 bcp = mk(0.0f, 0.0f, 0.0f);
   int bid = -1, ovf = 0;
@@ -1021,6 +1026,8 @@ static void bvh4_query(const orc_node4* nodes, const orc_tri4* tris, float delta
               sref[sp] = r[i];
               sd2[sp] = k[i];
               ++sp;
+              ++g_bvh4_pushes;
+              if (sp > g_bvh4_maxdepth) g_bvh4_maxdepth = sp;
             }
             cur = r[0];
             continue;
@@ -1063,6 +1070,19 @@ int orc_bvh4_query(const void* nodes, const void* tris, float delta, int cap, co
                thr_init ? thr_init[q] : FLT_MAX, cap, &d2[q], &c, &id[q], &ovf[q],
                nnodes ? &nnodes[q] : NULL, ntris ? &ntris[q] : NULL);
     vst(cp + 3 * q, c);
+  }
+  return 0;
+}
+
+/* Diagnostics: deepest stack and number of pushes of each query. */
+int orc_bvh4_stack_profile(const void* nodes, const void* tris, float delta, const float* pts, long n,
+                           const float* thr_init, int* max_depth, int* pushes) {
+  for (long q = 0; q < n; ++q) {
+    float d2; v3 c; int id, ovf; uint32_t nn, nt;
+    bvh4_query((const orc_node4*)nodes, (const orc_tri4*)tris, delta, vld(pts + 3 * q),
+               thr_init ? thr_init[q] : FLT_MAX, 16, &d2, &c, &id, &ovf, &nn, &nt);
+    max_depth[q] = g_bvh4_maxdepth;
+    pushes[q] = g_bvh4_pushes;
   }
   return 0;
 }

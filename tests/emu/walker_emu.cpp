@@ -38,6 +38,8 @@ long iters = 0;
 long trace = 0;
 int hung = 0;
 int diverged = 0;
+long path_iters[3];
+long path_lanes[3];
 
 inline void arrive(int op) {
   lanes[cur].op = op;
@@ -81,8 +83,14 @@ inline double __shfl_down(double v, int dlt) {
     break;                                                                                         \
   }
 #define N2_EMU_TRACE(sN, sT, sU)                                                                   \
-  if (lane == 0 && emu::trace > 0 && emu::iters <= emu::trace)                                     \
-    std::fprintf(stderr, "iter %ld sN %d sT %d sU %d\n", emu::iters, sN, sT, sU);
+  if (lane == 0) {                                                                                 \
+    const int path = ((sU) >= (sN) && (sU) >= (sT)) ? 2 : ((sN) >= (sT) ? 0 : 1);                  \
+    const int served = __builtin_popcountll(path == 2 ? mU : (path == 0 ? mN : mT));               \
+    emu::path_iters[path] += 1;                                                                    \
+    emu::path_lanes[path] += served;                                                               \
+    if (emu::trace > 0 && emu::iters <= emu::trace)                                                \
+      std::fprintf(stderr, "iter %ld sN %d sT %d sU %d\n", emu::iters, sN, sT, sU);               \
+  }
 
 namespace n2 {
 inline int lane_id() { return emu::cur; }
@@ -201,8 +209,15 @@ extern "C" int emu_walker(const void* nodes, const void* tris, float delta, cons
   emu::iters = 0;
   emu::hung = 0;
   emu::diverged = 0;
+  for (int k = 0; k < 3; ++k) emu::path_iters[k] = emu::path_lanes[k] = 0;
   return run_wave();
 }
 
 extern "C" long emu_iterations() { return emu::iters; }
 extern "C" void emu_set_trace(long n) { emu::trace = n; }
+extern "C" void emu_path_stats(long* iters, long* lanes) {
+  for (int k = 0; k < 3; ++k) {
+    iters[k] = emu::path_iters[k];
+    lanes[k] = emu::path_lanes[k];
+  }
+}
