@@ -209,6 +209,47 @@ int n2wos_wos_solve(const n2wos_bvh* bvh, const n2wos_boundary* boundary, const 
                     void* d_workspace, int64_t workspace_bytes, float* d_walk_values, int* d_walk_steps,
                     void* stream);
 
+/* ------------------------------------------------------------ learned cache */
+/* Multiresolution hash grid + ReLU MLP (2 n_levels inputs, 4 hidden layers of
+ * 64, 1 output), trained with Adam on the MSE loss.  Replaces the
+ * tiny-cuda-nn NetworkWithInputEncoding / Trainer / inference calls of
+ * src/tools/eval_tcnn_nc_wos.cu:747-769 (HashGrid + FullyFusedMLP, L2 loss,
+ * Adam).  MLP operands are FP16 on MFMA with FP32 accumulation and a static
+ * loss scale; tables and master weights are FP32.  World points map to the
+ * unit cube by (p - origin) / extent, clamped. */
+typedef struct n2wos_cache n2wos_cache;
+typedef struct {
+  int n_levels;          /* <= 16 */
+  int log2_table_size;   /* T = 2^log2_table_size entries per hashed level */
+  int base_resolution;   /* N_min */
+  float growth;          /* b: N_l = floor(N_min b^l) */
+  float origin[3];
+  float extent[3];
+  float learning_rate;   /* Adam */
+  float beta1, beta2, epsilon;
+  float loss_scale;      /* FP16 gradient scale */
+  uint64_t seed;         /* parameter initialisation */
+} n2wos_cache_config;
+int n2wos_cache_create(const n2wos_cache_config* config, n2wos_cache** out);
+int n2wos_cache_destroy(n2wos_cache* cache);
+/* d_out[i] = C(d_points[i]) for n points (12 B each). */
+int n2wos_cache_infer(const n2wos_cache* cache, const float* d_points, int n, float* d_out, void* stream);
+/* One Adam step on the batch (n points, n targets).  *d_loss_out (device,
+ * optional) receives the summed squared error.  The activation store grows
+ * (one allocation) the first time a larger batch is seen. */
+int n2wos_cache_train_step(n2wos_cache* cache, const float* d_points, const float* d_targets, int n,
+                           float* d_loss_out, void* stream);
+/* Grid layout: per-level offsets, sizes (entries), resolutions and dense flags. */
+int n2wos_cache_layout(const n2wos_cache* cache, int* n_levels, uint32_t* table_size, uint32_t* offsets,
+                       uint32_t* sizes, int* res, int* dense, uint32_t* total_entries);
+/* Host copies / uploads of the FP32 parameters (synchronous). */
+int n2wos_cache_get_params(const n2wos_cache* cache, float* grid, float* weights);
+int n2wos_cache_set_params(n2wos_cache* cache, const float* grid, const float* weights);
+/* Gradients of the mean squared error on one batch, without an update
+ * (synchronous; for gradient checks).  *loss = mean squared error. */
+int n2wos_cache_gradients(n2wos_cache* cache, const float* d_points, const float* d_targets, int n, float* grid_grad,
+                          float* weight_grad, float* loss);
+
 #ifdef __cplusplus
 }
 #endif

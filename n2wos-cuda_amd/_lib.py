@@ -74,6 +74,14 @@ _SIGNATURES = {
     "n2wos_bvh_export": ([vp, vp, vp], i32),
     "n2wos_bvh_inside_device": ([vp, vp, i32, vp, vp], i32),
     "n2wos_wos_workspace_size": ([i32, i32], i64),
+    "n2wos_cache_create": ([vp, C.POINTER(vp)], i32),
+    "n2wos_cache_destroy": ([vp], i32),
+    "n2wos_cache_infer": ([vp, vp, i32, vp, vp], i32),
+    "n2wos_cache_train_step": ([vp, vp, vp, i32, vp, vp], i32),
+    "n2wos_cache_layout": ([vp, vp, vp, vp, vp, vp, vp, vp], i32),
+    "n2wos_cache_get_params": ([vp, vp, vp], i32),
+    "n2wos_cache_set_params": ([vp, vp, vp], i32),
+    "n2wos_cache_gradients": ([vp, vp, vp, i32, vp, vp, vp], i32),
     "n2wos_wos_solve": ([vp, vp, vp, vp, i32, i32, C.c_uint32, vp, vp, vp, vp, vp, i32, vp, i64, vp, vp, vp], i32),
 }
 

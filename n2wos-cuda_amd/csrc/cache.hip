@@ -1,0 +1,738 @@
+// cache.hip -- learned cache kernels: hash-grid encoding, MLP forward and
+// backward on MFMA (FP16 operands, FP32 accumulation), weight gradients as
+// MFMA GEMMs over the batch, and Adam.  Layout and formulas: cache.hpp.
+//
+// MFMA fragment layout of v_mfma_f32_16x16x16_f16 (one wave, lane l, i = l % 16,
+// g = l / 16): A operand holds A[i][4g .. 4g+3], B operand B[4g .. 4g+3][i],
+// the accumulator D[4g .. 4g+3][i].  The MLP is evaluated transposed
+// (features x samples), so one layer's accumulator IS the next layer's B
+// operand: lane l holds features 4g..4g+3 of sample i of every 16-feature tile.
+// The hash-grid encoding is spread the same way: lane (i, g) encodes levels
+// 2g, 2g+1, 8+2g, 8+2g+1 of sample i, exactly the input features its B
+// fragments need, so no lane computes a level twice.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "api_internal.hpp"
+#include "cache.hpp"
+#include "geom.hpp"
+
+typedef _Float16 h4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+namespace n2 {
+
+namespace {
+
+__device__ __forceinline__ f4_t mfma16(h4_t a, h4_t b, f4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ h4_t relu_h4(f4_t v) {
+  h4_t r;
+  r[0] = static_cast<_Float16>(fmaxf(v[0], 0.0f));
+  r[1] = static_cast<_Float16>(fmaxf(v[1], 0.0f));
+  r[2] = static_cast<_Float16>(fmaxf(v[2], 0.0f));
+  r[3] = static_cast<_Float16>(fmaxf(v[3], 0.0f));
+  return r;
+}
+
+__device__ __forceinline__ f4_t zero4() {
+  f4_t z = {0.0f, 0.0f, 0.0f, 0.0f};
+  return z;
+}
+
+// Cell corner indices and trilinear weights of point u (unit cube) on level l.
+__host__ __device__ __forceinline__ void level_corners(const GridLayout& g, int l, float ux, float uy, float uz,
+                                                       uint32_t idx[8], float w[8]) {
+  const int n = g.res[l];
+  const float sx = ux * static_cast<float>(n), sy = uy * static_cast<float>(n), sz = uz * static_cast<float>(n);
+  int cx = static_cast<int>(floorf(sx)), cy = static_cast<int>(floorf(sy)), cz = static_cast<int>(floorf(sz));
+  cx = cx < 0 ? 0 : (cx > n - 1 ? n - 1 : cx);
+  cy = cy < 0 ? 0 : (cy > n - 1 ? n - 1 : cy);
+  cz = cz < 0 ? 0 : (cz > n - 1 ? n - 1 : cz);
+  const float fx = sx - static_cast<float>(cx), fy = sy - static_cast<float>(cy), fz = sz - static_cast<float>(cz);
+  const uint32_t stride = static_cast<uint32_t>(n + 1);
+  for (int c = 0; c < 8; ++c) {
+    const uint32_t x = static_cast<uint32_t>(cx + (c & 1));
+    const uint32_t y = static_cast<uint32_t>(cy + ((c >> 1) & 1));
+    const uint32_t z = static_cast<uint32_t>(cz + ((c >> 2) & 1));
+    uint32_t local;
+    if (g.dense[l])
+      local = x + y * stride + z * stride * stride;
+    else
+      local = (x ^ (y * 2654435761u) ^ (z * 805459861u)) & (g.table_size - 1u);
+    idx[c] = g.offset[l] + local;
+    w[c] = ((c & 1) ? fx : 1.0f - fx) * (((c >> 1) & 1) ? fy : 1.0f - fy) * (((c >> 2) & 1) ? fz : 1.0f - fz);
+  }
+}
+
+__host__ __device__ __forceinline__ void to_unit(const GridLayout& g, float px, float py, float pz, float& ux,
+                                                 float& uy, float& uz) {
+  ux = fminf(1.0f, fmaxf(0.0f, (px - g.in_min[0]) * g.in_scale[0]));
+  uy = fminf(1.0f, fmaxf(0.0f, (py - g.in_min[1]) * g.in_scale[1]));
+  uz = fminf(1.0f, fmaxf(0.0f, (pz - g.in_min[2]) * g.in_scale[2]));
+}
+
+// The two input fragments of lane (i, g): features of levels {2g, 2g+1} (t=0)
+// and {8+2g, 9+2g} (t=1), two features each.
+__device__ __forceinline__ void encode_fragments(const GridLayout& gl, const float* __restrict__ grid, float ux,
+                                                 float uy, float uz, int g, h4_t xb[2]) {
+  for (int t = 0; t < 2; ++t) {
+    float f[4];
+    for (int h = 0; h < 2; ++h) {
+      const int l = 8 * t + 2 * g + h;
+      float f0 = 0.0f, f1 = 0.0f;
+      if (l < gl.n_levels) {
+        uint32_t idx[8];
+        float w[8];
+        level_corners(gl, l, ux, uy, uz, idx, w);
+        for (int c = 0; c < 8; ++c) {
+          const float2 v = *reinterpret_cast<const float2*>(grid + 2 * static_cast<size_t>(idx[c]));
+          f0 += w[c] * v.x;
+          f1 += w[c] * v.y;
+        }
+      }
+      f[2 * h] = f0;
+      f[2 * h + 1] = f1;
+    }
+    xb[t][0] = static_cast<_Float16>(f[0]);
+    xb[t][1] = static_cast<_Float16>(f[1]);
+    xb[t][2] = static_cast<_Float16>(f[2]);
+    xb[t][3] = static_cast<_Float16>(f[3]);
+  }
+}
+
+__device__ __forceinline__ h4_t load_h4(const _Float16* p) { return *reinterpret_cast<const h4_t*>(p); }
+
+// A fragment of a row-major [rows x cols] FP16 matrix: row r0 + i, cols k0 + 4g ..
+__device__ __forceinline__ h4_t frag_rows(const _Float16* m, int cols, int r, int k) {
+  return load_h4(m + r * cols + k);
+}
+
+// A fragment of the TRANSPOSE of a row-major [rows x cols] matrix: element
+// (r, k) of M^T is M[k][r].
+__device__ __forceinline__ h4_t frag_trans(const _Float16* m, int cols, int r, int k) {
+  h4_t a;
+  a[0] = m[(k + 0) * cols + r];
+  a[1] = m[(k + 1) * cols + r];
+  a[2] = m[(k + 2) * cols + r];
+  a[3] = m[(k + 3) * cols + r];
+  return a;
+}
+
+// Forward of one 16-sample tile; returns the output accumulator (row 0 of the
+// padded output layer, held in element 0 of lanes with g == 0) and the
+// hidden activations.  Every MFMA here and below runs with the whole wave
+// active: the instruction reads all 64 lanes' operands.
+__device__ __forceinline__ f4_t mlp_forward(const _Float16* __restrict__ w, const h4_t xb[2], int i, int g,
+                                            h4_t h1[4], h4_t h2[4], h4_t h3[4], h4_t h4[4]) {
+  const _Float16* a1 = w;
+  const _Float16* a2 = w + kCacheOffA2;
+  const _Float16* a3 = w + kCacheOffA3;
+  const _Float16* a4 = w + kCacheOffA4;
+  const _Float16* a5 = w + kCacheOffA5;
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+    f4_t acc = zero4();
+#pragma unroll
+    for (int t = 0; t < 2; ++t) acc = mfma16(frag_rows(a1, kCacheIn, 16 * o + i, 16 * t + 4 * g), xb[t], acc);
+    h1[o] = relu_h4(acc);
+  }
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+    f4_t acc = zero4();
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc = mfma16(frag_rows(a2, kCacheHidden, 16 * o + i, 16 * t + 4 * g), h1[t], acc);
+    h2[o] = relu_h4(acc);
+  }
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+    f4_t acc = zero4();
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc = mfma16(frag_rows(a3, kCacheHidden, 16 * o + i, 16 * t + 4 * g), h2[t], acc);
+    h3[o] = relu_h4(acc);
+  }
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+    f4_t acc = zero4();
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc = mfma16(frag_rows(a4, kCacheHidden, 16 * o + i, 16 * t + 4 * g), h3[t], acc);
+    h4[o] = relu_h4(acc);
+  }
+  f4_t out = zero4();
+#pragma unroll
+  for (int t = 0; t < 4; ++t) out = mfma16(frag_rows(a5, kCacheHidden, i, 16 * t + 4 * g), h4[t], out);
+  return out;
+}
+
+constexpr int kCacheBlock = 256;
+
+__device__ __forceinline__ void stage_weights(_Float16* lds, const _Float16* __restrict__ w) {
+  for (int k = threadIdx.x; k < kCacheWeights / 4; k += blockDim.x)
+    reinterpret_cast<h4_t*>(lds)[k] = reinterpret_cast<const h4_t*>(w)[k];
+  __syncthreads();
+}
+
+// Inference: out[s] = C(points[s]) for s < n.
+__global__ void __launch_bounds__(kCacheBlock) k_cache_infer(GridLayout gl, const float* __restrict__ grid,
+                                                            const _Float16* __restrict__ wh,
+                                                            const float* __restrict__ points, int n,
+                                                            float* __restrict__ out) {
+  __shared__ _Float16 w[kCacheWeights];
+  stage_weights(w, wh);
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const int i = lane & 15, g = lane >> 4;
+  const int waves = static_cast<int>(gridDim.x * blockDim.x / 64);
+  const int wave = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / 64);
+  const int tiles = (n + 15) / 16;
+  for (int tile = wave; tile < tiles; tile += waves) {
+    const int s = tile * 16 + i;
+    float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+    if (s < n) to_unit(gl, points[3 * s], points[3 * s + 1], points[3 * s + 2], ux, uy, uz);
+    h4_t xb[2];
+    encode_fragments(gl, grid, ux, uy, uz, g, xb);
+    h4_t h1[4], h2[4], h3[4], h4[4];
+    const f4_t y = mlp_forward(w, xb, i, g, h1, h2, h3, h4);
+    if (g == 0 && s < n) out[s] = y[0];
+  }
+}
+
+// Training pass 1: forward + backward of every 16-sample tile.  Stores the
+// layer inputs X, H1..H4 and the pre-activation gradients dZ1..dZ4, dY (FP16,
+// [feature][batch]) for the weight-gradient GEMMs, scatters the input-feature
+// gradients into the grid gradient, and adds the loss to *loss_sum.
+struct TrainStore {
+  _Float16* x;    // [32][B]
+  _Float16* h;    // [4][64][B]  H1..H4
+  _Float16* dz;   // [4][64][B]  dZ1..dZ4
+  _Float16* dy;   // [B]         dL/dy * loss_scale
+  int batch;      // B (multiple of 16)
+};
+
+__device__ __forceinline__ void store_frag(_Float16* base, int batch, int feat0, int s, h4_t v) {
+  base[static_cast<size_t>(feat0 + 0) * batch + s] = v[0];
+  base[static_cast<size_t>(feat0 + 1) * batch + s] = v[1];
+  base[static_cast<size_t>(feat0 + 2) * batch + s] = v[2];
+  base[static_cast<size_t>(feat0 + 3) * batch + s] = v[3];
+}
+
+__device__ __forceinline__ h4_t mask_relu(f4_t d, h4_t h, float& dummy) {
+  (void)dummy;
+  h4_t r;
+  r[0] = static_cast<_Float16>(static_cast<float>(h[0]) > 0.0f ? d[0] : 0.0f);
+  r[1] = static_cast<_Float16>(static_cast<float>(h[1]) > 0.0f ? d[1] : 0.0f);
+  r[2] = static_cast<_Float16>(static_cast<float>(h[2]) > 0.0f ? d[2] : 0.0f);
+  r[3] = static_cast<_Float16>(static_cast<float>(h[3]) > 0.0f ? d[3] : 0.0f);
+  return r;
+}
+
+__global__ void __launch_bounds__(kCacheBlock) k_cache_train_pass1(GridLayout gl, const float* __restrict__ grid,
+                                                                  const _Float16* __restrict__ wh,
+                                                                  const float* __restrict__ points,
+                                                                  const float* __restrict__ targets, int n,
+                                                                  float grad_scale, TrainStore st,
+                                                                  float* __restrict__ grid_grad,
+                                                                  float* __restrict__ loss_sum) {
+  __shared__ _Float16 w[kCacheWeights];
+  stage_weights(w, wh);
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const int i = lane & 15, g = lane >> 4;
+  const int waves = static_cast<int>(gridDim.x * blockDim.x / 64);
+  const int wave = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / 64);
+  const int tiles = st.batch / 16;
+  const _Float16* a1 = w;
+  const _Float16* a2 = w + kCacheOffA2;
+  const _Float16* a3 = w + kCacheOffA3;
+  const _Float16* a4 = w + kCacheOffA4;
+  const _Float16* a5 = w + kCacheOffA5;
+  float loss = 0.0f;
+  for (int tile = wave; tile < tiles; tile += waves) {
+    const int s = tile * 16 + i;
+    const bool live = s < n;
+    float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+    if (live) to_unit(gl, points[3 * s], points[3 * s + 1], points[3 * s + 2], ux, uy, uz);
+    h4_t xb[2];
+    encode_fragments(gl, grid, ux, uy, uz, g, xb);
+    h4_t h1[4], h2[4], h3[4], h4[4];
+    const f4_t y = mlp_forward(w, xb, i, g, h1, h2, h3, h4);
+    // dL/dy for L = mean (y - t)^2, scaled; broadcast from lane (i, 0) to (i, g)
+    float dy = 0.0f;
+    if (g == 0 && live) {
+      const float diff = y[0] - targets[s];
+      loss += diff * diff;
+      dy = 2.0f * diff * grad_scale;
+    }
+    dy = __shfl(dy, i);
+    // stores for the weight gradients
+    for (int t = 0; t < 2; ++t) store_frag(st.x, st.batch, 16 * t + 4 * g, s, xb[t]);
+    for (int o = 0; o < 4; ++o) {
+      store_frag(st.h + 0 * 64 * static_cast<size_t>(st.batch), st.batch, 16 * o + 4 * g, s, h1[o]);
+      store_frag(st.h + 1 * 64 * static_cast<size_t>(st.batch), st.batch, 16 * o + 4 * g, s, h2[o]);
+      store_frag(st.h + 2 * 64 * static_cast<size_t>(st.batch), st.batch, 16 * o + 4 * g, s, h3[o]);
+      store_frag(st.h + 3 * 64 * static_cast<size_t>(st.batch), st.batch, 16 * o + 4 * g, s, h4[o]);
+    }
+    if (g == 0) st.dy[s] = static_cast<_Float16>(dy);
+    // output layer: dH4 = A5^T dY, dY = [dy; 0; ...] (16 x 16)
+    h4_t bdy;
+    bdy[0] = static_cast<_Float16>(g == 0 ? dy : 0.0f);
+    bdy[1] = bdy[2] = bdy[3] = static_cast<_Float16>(0.0f);
+    float dmy = 0.0f;
+    h4_t dz4[4], dz3[4], dz2[4], dz1[4];
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+      f4_t acc = mfma16(frag_trans(a5, kCacheHidden, 16 * o + i, 4 * g), bdy, zero4());
+      dz4[o] = mask_relu(acc, h4[o], dmy);
+      store_frag(st.dz + 3 * 64 * static_cast<size_t>(st.batch), st.batch, 16 * o + 4 * g, s, dz4[o]);
+    }
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+      f4_t acc = zero4();
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc = mfma16(frag_trans(a4, kCacheHidden, 16 * o + i, 16 * t + 4 * g), dz4[t], acc);
+      dz3[o] = mask_relu(acc, h3[o], dmy);
+      store_frag(st.dz + 2 * 64 * static_cast<size_t>(st.batch), st.batch, 16 * o + 4 * g, s, dz3[o]);
+    }
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+      f4_t acc = zero4();
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc = mfma16(frag_trans(a3, kCacheHidden, 16 * o + i, 16 * t + 4 * g), dz3[t], acc);
+      dz2[o] = mask_relu(acc, h2[o], dmy);
+      store_frag(st.dz + 1 * 64 * static_cast<size_t>(st.batch), st.batch, 16 * o + 4 * g, s, dz2[o]);
+    }
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+      f4_t acc = zero4();
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc = mfma16(frag_trans(a2, kCacheHidden, 16 * o + i, 16 * t + 4 * g), dz2[t], acc);
+      dz1[o] = mask_relu(acc, h1[o], dmy);
+      store_frag(st.dz + 0 * 64 * static_cast<size_t>(st.batch), st.batch, 16 * o + 4 * g, s, dz1[o]);
+    }
+    // input-feature gradients dX = A1^T dZ1 (32 x 16), computed by the whole
+    // wave (an MFMA reads every lane's operands), then scattered to the grid
+    // by the lanes that hold a real sample
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      f4_t acc = zero4();
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc = mfma16(frag_trans(a1, kCacheIn, 16 * t + i, 16 * k + 4 * g), dz1[k], acc);
+      if (live) {
+        for (int h = 0; h < 2; ++h) {
+          const int l = 8 * t + 2 * g + h;
+          if (l >= gl.n_levels) continue;
+          const float g0 = acc[2 * h], g1 = acc[2 * h + 1];
+          if (g0 == 0.0f && g1 == 0.0f) continue;
+          uint32_t idx[8];
+          float wc[8];
+          level_corners(gl, l, ux, uy, uz, idx, wc);
+          for (int c = 0; c < 8; ++c) {
+            atomicAdd(grid_grad + 2 * static_cast<size_t>(idx[c]), wc[c] * g0);
+            atomicAdd(grid_grad + 2 * static_cast<size_t>(idx[c]) + 1, wc[c] * g1);
+          }
+        }
+      }
+    }
+  }
+  // loss: sum over the wave's lanes (only g == 0 lanes carry values)
+  for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+  if (lane == 0) atomicAdd(loss_sum, loss);
+}
+
+// Training pass 2: weight gradients dA_l = dZ_l H_{l-1}^T over the batch
+// (K = samples) on MFMA.  blockIdx.y = layer (0..4); each wave reduces a
+// K-chunk into ROWS_T x COLS_T accumulator tiles and adds them to the FP32
+// gradient with atomics.  ROWS_T = 0 marks the output layer, whose dZ is the
+// single row dY (rows 1..15 of the padded tile are zero).
+template <int ROWS_T, int COLS_T>
+__device__ __forceinline__ void wgrad_layer(const _Float16* __restrict__ dz, const _Float16* __restrict__ hp,
+                                            size_t B, int k0, int k1, int i, int g, int cols, float* __restrict__ wg) {
+  constexpr int R = ROWS_T == 0 ? 1 : ROWS_T;
+  f4_t acc[R * COLS_T];
+#pragma unroll
+  for (int q = 0; q < R * COLS_T; ++q) acc[q] = zero4();
+  for (int k = k0; k < k1; k += 16) {
+    h4_t bf[COLS_T];
+#pragma unroll
+    for (int c = 0; c < COLS_T; ++c) bf[c] = load_h4(hp + static_cast<size_t>(16 * c + i) * B + k + 4 * g);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      h4_t af;
+      if (ROWS_T == 0) {
+        const h4_t v = load_h4(dz + k + 4 * g);
+        const _Float16 z = static_cast<_Float16>(0.0f);
+        af[0] = i == 0 ? v[0] : z;
+        af[1] = i == 0 ? v[1] : z;
+        af[2] = i == 0 ? v[2] : z;
+        af[3] = i == 0 ? v[3] : z;
+      } else {
+        af = load_h4(dz + static_cast<size_t>(16 * r + i) * B + k + 4 * g);
+      }
+#pragma unroll
+      for (int c = 0; c < COLS_T; ++c) acc[R * 0 + r * COLS_T + c] = mfma16(af, bf[c], acc[r * COLS_T + c]);
+    }
+  }
+  // acc[r * COLS_T + c] holds D[4g + e][i] of tile (r, c): row 16r + 4g + e, col 16c + i
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int c = 0; c < COLS_T; ++c)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = 16 * r + 4 * g + e;
+        if (ROWS_T == 0 && row != 0) continue;
+        atomicAdd(wg + row * cols + 16 * c + i, acc[r * COLS_T + c][e]);
+      }
+}
+
+__global__ void __launch_bounds__(kCacheBlock) k_cache_wgrad(TrainStore st, int chunk, float* __restrict__ w_grad) {
+  const int layer = static_cast<int>(blockIdx.y);
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const int i = lane & 15, g = lane >> 4;
+  const int wave = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / 64);
+  const int k0 = wave * chunk;
+  if (k0 >= st.batch) return;
+  const int k1 = min(st.batch, k0 + chunk);
+  const size_t B = static_cast<size_t>(st.batch);
+  if (layer == 0) {
+    wgrad_layer<4, 2>(st.dz, st.x, B, k0, k1, i, g, kCacheIn, w_grad);
+  } else if (layer < 4) {
+    wgrad_layer<4, 4>(st.dz + layer * 64 * B, st.h + (layer - 1) * 64 * B, B, k0, k1, i, g, kCacheHidden,
+                      w_grad + kCacheW1 + (layer - 1) * kCacheWh);
+  } else {
+    wgrad_layer<0, 4>(st.dy, st.h + 3 * 64 * B, B, k0, k1, i, g, kCacheHidden, w_grad + kCacheOffA5);
+  }
+}
+
+// Adam (Kingma & Ba) on FP32 parameters with unscaled gradients; zeroes the
+// gradient.  Skips the update (and still zeroes) when *bad is set.
+__global__ void k_adam(float* __restrict__ p, float* __restrict__ grad, float* __restrict__ m, float* __restrict__ v,
+                       size_t n, float lr_t, float beta1, float beta2, float eps, float inv_scale,
+                       const int* __restrict__ bad) {
+  const bool skip = *bad != 0;
+  for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; k < n;
+       k += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const float gk = grad[k] * inv_scale;
+    grad[k] = 0.0f;
+    if (skip) continue;
+    const float mk = beta1 * m[k] + (1.0f - beta1) * gk;
+    const float vk = beta2 * v[k] + (1.0f - beta2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    p[k] = p[k] - lr_t * mk / (sqrtf(vk) + eps);
+  }
+}
+
+__global__ void k_check_finite(const float* __restrict__ a, size_t n, int* __restrict__ bad) {
+  for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; k < n;
+       k += static_cast<size_t>(gridDim.x) * blockDim.x)
+    if (!isfinite(a[k])) *bad = 1;
+}
+
+__global__ void k_to_half(const float* __restrict__ a, _Float16* __restrict__ h, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) h[k] = static_cast<_Float16>(a[k]);
+}
+
+__global__ void k_zero_int(int* p) { *p = 0; }
+
+__global__ void k_init_uniform(float* __restrict__ p, size_t n, float lo, float hi, uint32_t seed, uint32_t tag) {
+  for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; k < n;
+       k += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const U4 r = philox4x32_10(U4{static_cast<uint32_t>(k), static_cast<uint32_t>(k >> 32), tag, 0x63616368u}, seed,
+                               0x9e3779b9u);
+    p[k] = lo + (hi - lo) * u01(r.x);
+  }
+}
+
+}  // namespace
+}  // namespace n2
+
+// ------------------------------------------------------------------ handle
+struct n2wos_cache {
+  n2::GridLayout layout{};
+  float* grid = nullptr;
+  float* grid_grad = nullptr;
+  float* grid_m = nullptr;
+  float* grid_v = nullptr;
+  float* w = nullptr;
+  float* w_grad = nullptr;
+  float* w_m = nullptr;
+  float* w_v = nullptr;
+  _Float16* wh = nullptr;
+  int* bad = nullptr;
+  float* loss = nullptr;
+  _Float16* store = nullptr;
+  int store_batch = 0;
+  int64_t step = 0;
+  float lr = 1e-2f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-15f, loss_scale = 128.0f;
+};
+
+using namespace n2;
+
+namespace {
+
+size_t grid_floats(const n2wos_cache* c) { return 2 * static_cast<size_t>(c->layout.total_entries); }
+
+void free_cache(n2wos_cache* c) {
+  for (void* p : {static_cast<void*>(c->grid), static_cast<void*>(c->grid_grad), static_cast<void*>(c->grid_m),
+                  static_cast<void*>(c->grid_v), static_cast<void*>(c->w), static_cast<void*>(c->w_grad),
+                  static_cast<void*>(c->w_m), static_cast<void*>(c->w_v), static_cast<void*>(c->wh),
+                  static_cast<void*>(c->bad), static_cast<void*>(c->loss), static_cast<void*>(c->store)})
+    if (p) (void)hipFree(p);
+  delete c;
+}
+
+}  // namespace
+
+extern "C" int n2wos_cache_create(const n2wos_cache_config* cfg, n2wos_cache** out) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(cfg && out, N2WOS_ERR_NULL, "NULL argument");
+    *out = nullptr;
+    N2_REQUIRE(cfg->n_levels >= 1 && cfg->n_levels <= kCacheLevels, N2WOS_ERR_INVALID, "n_levels must be in [1, 16]");
+    N2_REQUIRE(cfg->log2_table_size >= 8 && cfg->log2_table_size <= 24, N2WOS_ERR_INVALID,
+               "log2_table_size must be in [8, 24]");
+    N2_REQUIRE(cfg->base_resolution >= 1 && cfg->growth >= 1.0f, N2WOS_ERR_INVALID, "bad resolution parameters");
+    for (int a = 0; a < 3; ++a) N2_REQUIRE(cfg->extent[a] > 0.0f, N2WOS_ERR_INVALID, "extent must be positive");
+    auto* c = new n2wos_cache();
+    try {
+      GridLayout& L = c->layout;
+      L.n_levels = cfg->n_levels;
+      L.table_size = 1u << cfg->log2_table_size;
+      uint64_t off = 0;
+      for (int l = 0; l < kCacheLevels; ++l) {
+        const double nres = std::floor(static_cast<double>(cfg->base_resolution) * std::pow(cfg->growth, l));
+        L.res[l] = static_cast<int>(nres);
+        const double verts = std::pow(nres + 1.0, 3.0);
+        L.dense[l] = verts <= static_cast<double>(L.table_size) ? 1 : 0;
+        L.size[l] = L.dense[l] ? static_cast<uint32_t>((static_cast<uint64_t>(verts) + 7) / 8 * 8) : L.table_size;
+        L.offset[l] = static_cast<uint32_t>(off);
+        if (l < L.n_levels) off += L.size[l];
+      }
+      N2_REQUIRE(off < (uint64_t{1} << 31), N2WOS_ERR_INVALID, "grid too large");
+      L.total_entries = static_cast<uint32_t>(off);
+      for (int a = 0; a < 3; ++a) {
+        L.in_min[a] = cfg->origin[a];
+        L.in_scale[a] = 1.0f / cfg->extent[a];
+      }
+      c->lr = cfg->learning_rate;
+      c->beta1 = cfg->beta1;
+      c->beta2 = cfg->beta2;
+      c->eps = cfg->epsilon;
+      c->loss_scale = cfg->loss_scale;
+      const size_t gf = grid_floats(c);
+      N2_HIP(hipMalloc(&c->grid, gf * 4));
+      N2_HIP(hipMalloc(&c->grid_grad, gf * 4));
+      N2_HIP(hipMalloc(&c->grid_m, gf * 4));
+      N2_HIP(hipMalloc(&c->grid_v, gf * 4));
+      N2_HIP(hipMalloc(&c->w, kCacheWeights * 4));
+      N2_HIP(hipMalloc(&c->w_grad, kCacheWeights * 4));
+      N2_HIP(hipMalloc(&c->w_m, kCacheWeights * 4));
+      N2_HIP(hipMalloc(&c->w_v, kCacheWeights * 4));
+      N2_HIP(hipMalloc(&c->wh, kCacheWeights * 2));
+      N2_HIP(hipMalloc(&c->bad, 64));
+      N2_HIP(hipMalloc(&c->loss, 64));
+      N2_HIP(hipMemset(c->grid_grad, 0, gf * 4));
+      N2_HIP(hipMemset(c->grid_m, 0, gf * 4));
+      N2_HIP(hipMemset(c->grid_v, 0, gf * 4));
+      N2_HIP(hipMemset(c->w_grad, 0, kCacheWeights * 4));
+      N2_HIP(hipMemset(c->w_m, 0, kCacheWeights * 4));
+      N2_HIP(hipMemset(c->w_v, 0, kCacheWeights * 4));
+      N2_HIP(hipMemset(c->bad, 0, 64));
+      // grid: uniform +-1e-4; MLP: Xavier-uniform per layer (Philox, seeded)
+      hipLaunchKernelGGL(k_init_uniform, dim3(1024), dim3(256), 0, nullptr, c->grid, gf, -1e-4f, 1e-4f,
+                         static_cast<uint32_t>(cfg->seed), 0u);
+      const int fan[5][3] = {{kCacheIn, kCacheHidden, 0}, {kCacheHidden, kCacheHidden, kCacheOffA2},
+                             {kCacheHidden, kCacheHidden, kCacheOffA3}, {kCacheHidden, kCacheHidden, kCacheOffA4},
+                             {kCacheHidden, 1, kCacheOffA5}};
+      for (int l = 0; l < 5; ++l) {
+        const float a = std::sqrt(6.0f / static_cast<float>(fan[l][0] + fan[l][1]));
+        const size_t cnt = static_cast<size_t>(fan[l][0]) * fan[l][1];
+        hipLaunchKernelGGL(k_init_uniform, dim3(8), dim3(256), 0, nullptr, c->w + fan[l][2], cnt, -a, a,
+                           static_cast<uint32_t>(cfg->seed), static_cast<uint32_t>(l + 1));
+      }
+      N2_HIP(hipMemset(c->w + kCacheOffA5 + kCacheHidden, 0, (kCacheW5 - kCacheHidden) * 4));
+      hipLaunchKernelGGL(k_to_half, dim3((kCacheWeights + 255) / 256), dim3(256), 0, nullptr, c->w, c->wh,
+                         kCacheWeights);
+      N2_HIP(hipGetLastError());
+      N2_HIP(hipDeviceSynchronize());
+    } catch (...) {
+      free_cache(c);
+      throw;
+    }
+    *out = c;
+    return N2WOS_OK;
+  });
+}
+
+extern "C" int n2wos_cache_destroy(n2wos_cache* c) {
+  return guarded([&]() -> int {
+    if (c) free_cache(c);
+    return N2WOS_OK;
+  });
+}
+
+extern "C" int n2wos_cache_infer(const n2wos_cache* c, const float* d_points, int n, float* d_out, void* stream) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(c != nullptr, N2WOS_ERR_NULL, "NULL cache");
+    N2_REQUIRE(n >= 0, N2WOS_ERR_INVALID, "n must be non-negative");
+    if (n == 0) return N2WOS_OK;
+    N2_REQUIRE(d_points && d_out, N2WOS_ERR_NULL, "NULL device pointer");
+    const int tiles = (n + 15) / 16;
+    const int blocks = std::max(1, std::min(4096, (tiles + 3) / 4));
+    hipLaunchKernelGGL(k_cache_infer, dim3(blocks), dim3(kCacheBlock), 0, static_cast<hipStream_t>(stream),
+                       c->layout, c->grid, c->wh, d_points, n, d_out);
+    N2_HIP(hipGetLastError());
+    return N2WOS_OK;
+  });
+}
+
+extern "C" int n2wos_cache_train_step(n2wos_cache* c, const float* d_points, const float* d_targets, int n,
+                                      float* d_loss_out, void* stream) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(c != nullptr, N2WOS_ERR_NULL, "NULL cache");
+    N2_REQUIRE(n >= 1, N2WOS_ERR_INVALID, "batch must be >= 1");
+    N2_REQUIRE(d_points && d_targets, N2WOS_ERR_NULL, "NULL device pointer");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int batch = (n + 15) / 16 * 16;
+    if (batch > c->store_batch) {
+      // grow the activation store (the one allocation of the training path)
+      if (c->store) N2_HIP(hipFree(c->store));
+      c->store = nullptr;
+      const size_t halves = static_cast<size_t>(batch) * (kCacheIn + 8 * kCacheHidden + 1);
+      N2_HIP(hipMalloc(&c->store, halves * 2));
+      c->store_batch = batch;
+    }
+    TrainStore st;
+    st.batch = batch;
+    st.x = c->store;
+    st.h = st.x + static_cast<size_t>(kCacheIn) * batch;
+    st.dz = st.h + static_cast<size_t>(4 * kCacheHidden) * batch;
+    st.dy = st.dz + static_cast<size_t>(4 * kCacheHidden) * batch;
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, c->bad);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, reinterpret_cast<int*>(c->loss));
+    const float grad_scale = c->loss_scale / static_cast<float>(n);
+    const int tiles = batch / 16;
+    const int blocks = std::max(1, std::min(2048, (tiles + 3) / 4));
+    hipLaunchKernelGGL(k_cache_train_pass1, dim3(blocks), dim3(kCacheBlock), 0, s, c->layout, c->grid, c->wh,
+                       d_points, d_targets, n, grad_scale, st, c->grid_grad, c->loss);
+    const int chunk = 1024;
+    const int waves = (batch + chunk - 1) / chunk;
+    hipLaunchKernelGGL(k_cache_wgrad, dim3((waves + 3) / 4, 5), dim3(kCacheBlock), 0, s, st, chunk, c->w_grad);
+    hipLaunchKernelGGL(k_check_finite, dim3(64), dim3(256), 0, s, c->w_grad, static_cast<size_t>(kCacheWeights),
+                       c->bad);
+    hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, s, c->grid_grad, grid_floats(c), c->bad);
+    c->step += 1;
+    const double bc1 = 1.0 - std::pow(static_cast<double>(c->beta1), static_cast<double>(c->step));
+    const double bc2 = 1.0 - std::pow(static_cast<double>(c->beta2), static_cast<double>(c->step));
+    const float lr_t = static_cast<float>(c->lr * std::sqrt(bc2) / bc1);
+    const float eps_t = static_cast<float>(c->eps * std::sqrt(bc2));
+    const float inv = 1.0f / c->loss_scale;
+    hipLaunchKernelGGL(k_adam, dim3(64), dim3(256), 0, s, c->w, c->w_grad, c->w_m, c->w_v,
+                       static_cast<size_t>(kCacheWeights), lr_t, c->beta1, c->beta2, eps_t, inv, c->bad);
+    hipLaunchKernelGGL(k_adam, dim3(2048), dim3(256), 0, s, c->grid, c->grid_grad, c->grid_m, c->grid_v,
+                       grid_floats(c), lr_t, c->beta1, c->beta2, eps_t, inv, c->bad);
+    hipLaunchKernelGGL(k_to_half, dim3((kCacheWeights + 255) / 256), dim3(256), 0, s, c->w, c->wh, kCacheWeights);
+    N2_HIP(hipGetLastError());
+    if (d_loss_out) N2_HIP(hipMemcpyAsync(d_loss_out, c->loss, 4, hipMemcpyDeviceToDevice, s));
+    return N2WOS_OK;
+  });
+}
+
+extern "C" int n2wos_cache_layout(const n2wos_cache* c, int* n_levels, uint32_t* table_size, uint32_t* offsets,
+                                  uint32_t* sizes, int* res, int* dense, uint32_t* total_entries) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(c != nullptr, N2WOS_ERR_NULL, "NULL cache");
+    const GridLayout& L = c->layout;
+    if (n_levels) *n_levels = L.n_levels;
+    if (table_size) *table_size = L.table_size;
+    for (int l = 0; l < kCacheLevels; ++l) {
+      if (offsets) offsets[l] = L.offset[l];
+      if (sizes) sizes[l] = L.size[l];
+      if (res) res[l] = L.res[l];
+      if (dense) dense[l] = L.dense[l];
+    }
+    if (total_entries) *total_entries = L.total_entries;
+    return N2WOS_OK;
+  });
+}
+
+// Host copies of (or host uploads to) the FP32 parameters: grid [2 * entries],
+// MLP weights [15360] (layout in the header); either pointer may be NULL.
+extern "C" int n2wos_cache_get_params(const n2wos_cache* c, float* grid, float* weights) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(c != nullptr, N2WOS_ERR_NULL, "NULL cache");
+    N2_HIP(hipDeviceSynchronize());
+    if (grid) N2_HIP(hipMemcpy(grid, c->grid, grid_floats(c) * 4, hipMemcpyDeviceToHost));
+    if (weights) N2_HIP(hipMemcpy(weights, c->w, kCacheWeights * 4, hipMemcpyDeviceToHost));
+    return N2WOS_OK;
+  });
+}
+
+extern "C" int n2wos_cache_set_params(n2wos_cache* c, const float* grid, const float* weights) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(c != nullptr, N2WOS_ERR_NULL, "NULL cache");
+    N2_HIP(hipDeviceSynchronize());
+    if (grid) N2_HIP(hipMemcpy(c->grid, grid, grid_floats(c) * 4, hipMemcpyHostToDevice));
+    if (weights) {
+      N2_HIP(hipMemcpy(c->w, weights, kCacheWeights * 4, hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(k_to_half, dim3((kCacheWeights + 255) / 256), dim3(256), 0, nullptr, c->w, c->wh,
+                         kCacheWeights);
+      N2_HIP(hipGetLastError());
+    }
+    N2_HIP(hipDeviceSynchronize());
+    return N2WOS_OK;
+  });
+}
+
+// Gradients of the last training step are consumed by Adam; this entry point
+// computes them WITHOUT updating (for gradient checks): pass 1 and pass 2 into
+// the gradient buffers, copied to the host (unscaled), then zeroed.
+extern "C" int n2wos_cache_gradients(n2wos_cache* c, const float* d_points, const float* d_targets, int n,
+                                     float* grid_grad, float* weight_grad, float* loss) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(c != nullptr, N2WOS_ERR_NULL, "NULL cache");
+    N2_REQUIRE(n >= 1 && d_points && d_targets, N2WOS_ERR_INVALID, "bad batch");
+    const int batch = (n + 15) / 16 * 16;
+    if (batch > c->store_batch) {
+      if (c->store) N2_HIP(hipFree(c->store));
+      c->store = nullptr;
+      N2_HIP(hipMalloc(&c->store, static_cast<size_t>(batch) * (kCacheIn + 8 * kCacheHidden + 1) * 2));
+      c->store_batch = batch;
+    }
+    TrainStore st;
+    st.batch = batch;
+    st.x = c->store;
+    st.h = st.x + static_cast<size_t>(kCacheIn) * batch;
+    st.dz = st.h + static_cast<size_t>(4 * kCacheHidden) * batch;
+    st.dy = st.dz + static_cast<size_t>(4 * kCacheHidden) * batch;
+    N2_HIP(hipMemset(c->loss, 0, 4));
+    const int tiles = batch / 16;
+    hipLaunchKernelGGL(k_cache_train_pass1, dim3(std::max(1, std::min(2048, (tiles + 3) / 4))), dim3(kCacheBlock), 0,
+                       nullptr, c->layout, c->grid, c->wh, d_points, d_targets, n,
+                       c->loss_scale / static_cast<float>(n), st, c->grid_grad, c->loss);
+    const int waves = (batch + 1023) / 1024;
+    hipLaunchKernelGGL(k_cache_wgrad, dim3((waves + 3) / 4, 5), dim3(kCacheBlock), 0, nullptr, st, 1024, c->w_grad);
+    N2_HIP(hipGetLastError());
+    N2_HIP(hipDeviceSynchronize());
+    const size_t gf = grid_floats(c);
+    std::vector<float> tmp;
+    if (grid_grad) N2_HIP(hipMemcpy(grid_grad, c->grid_grad, gf * 4, hipMemcpyDeviceToHost));
+    if (weight_grad) N2_HIP(hipMemcpy(weight_grad, c->w_grad, kCacheWeights * 4, hipMemcpyDeviceToHost));
+    const float inv = 1.0f / c->loss_scale;
+    if (grid_grad)
+      for (size_t k = 0; k < gf; ++k) grid_grad[k] *= inv;
+    if (weight_grad)
+      for (int k = 0; k < kCacheWeights; ++k) weight_grad[k] *= inv;
+    if (loss) {
+      N2_HIP(hipMemcpy(loss, c->loss, 4, hipMemcpyDeviceToHost));
+      *loss /= static_cast<float>(n);
+    }
+    N2_HIP(hipMemset(c->grid_grad, 0, gf * 4));
+    N2_HIP(hipMemset(c->w_grad, 0, kCacheWeights * 4));
+    return N2WOS_OK;
+  });
+}

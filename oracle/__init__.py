@@ -84,6 +84,13 @@ def lib():
         L.orc_point_stats.argtypes = [_f32p, C.c_long, C.c_int, _f64p, _f64p, _f64p]
         L.orc_bvh4_query.argtypes = [C.c_void_p, C.c_void_p, C.c_float, C.c_int, _f32p, C.c_long, C.c_void_p,
                                      _f32p, _f32p, _i32p, _i32p, C.c_void_p, C.c_void_p]
+        L.orc_cache_layout.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p]
+        L.orc_cache_forward.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                        C.c_long, _f64p]
+        L.orc_cache_grads.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                      _f32p, C.c_long, _f64p, _f64p]
+        L.orc_cache_grads.restype = C.c_double
         _lib = L
     return _lib
 
@@ -253,3 +260,37 @@ def bvh4_query(nodes, recs, delta, pts, cap=16, thr_init=None):
     if rc != 0:
         raise ValueError("bad ring capacity")
     return d2, cp, tri, ovf, nn, nt
+
+
+def cache_layout(n_levels, log2_t, base_res, growth):
+    off = np.zeros(16, np.uint32)
+    sz = np.zeros(16, np.uint32)
+    res = np.zeros(16, np.int32)
+    dense = np.zeros(16, np.int32)
+    tot = np.zeros(1, np.uint32)
+    lib().orc_cache_layout(n_levels, log2_t, base_res, growth, off.ctypes.data, sz.ctypes.data, res.ctypes.data,
+                           dense.ctypes.data, tot.ctypes.data)
+    return {"offsets": off.tolist(), "sizes": sz.tolist(), "res": res.tolist(), "dense": dense.tolist(),
+            "total_entries": int(tot[0])}
+
+
+def _cfg_args(cfg):
+    return (cfg.n_levels, cfg.log2_table_size, cfg.base_resolution, float(np.float32(cfg.growth)),
+            _f32(cfg.origin), _f32(cfg.extent))
+
+
+def cache_forward(cfg, grid, weights, pts):
+    """FP64 forward of the cache at (n, 3) points (cfg: an n2wos_amd CacheConfig)."""
+    pts = _f32(pts).reshape(-1, 3)
+    out = np.zeros(len(pts))
+    lib().orc_cache_forward(*_cfg_args(cfg), _f32(grid), _f32(weights), pts, len(pts), out)
+    return out
+
+
+def cache_grads(cfg, grid, weights, pts, targets):
+    """FP64 mean-squared-error gradients: (grid_grad, weight_grad, loss)."""
+    pts = _f32(pts).reshape(-1, 3)
+    gg = np.zeros(len(grid))
+    wg = np.zeros(len(weights))
+    loss = lib().orc_cache_grads(*_cfg_args(cfg), _f32(grid), _f32(weights), pts, _f32(targets), len(pts), gg, wg)
+    return gg, wg, loss

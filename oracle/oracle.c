@@ -1086,3 +1086,170 @@ int orc_bvh4_stack_profile(const void* nodes, const void* tris, float delta, con
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* Learned-cache oracle in FP64: multiresolution hash-grid encoding (Mueller et
+ * al. 2022: N_l = floor(N_min b^l), dense levels when (N_l + 1)^3 <= T,
+ * spatial hash x*1 ^ y*2654435761 ^ z*805459861 mod T, trilinear weights) and
+ * a bias-free ReLU MLP 2L -> 64 -> 64 -> 64 -> 64 -> 1, with the gradients of
+ * the mean squared error by reverse accumulation.  Weight buffer layout:
+ * A1 [64 x 2L] (2L = 32 columns reserved), A2..A4 [64 x 64], A5 row 0 [64]
+ * (rows 1..15 of a 16 x 64 block unused), offsets 0, 2048, 6144, 10240, 14336. */
+typedef struct {
+  int n_levels;
+  uint32_t table_size;
+  uint32_t offset[16], size[16];
+  int res[16], dense[16];
+  uint32_t total;
+  double in_min[3], in_scale[3];
+} orc_grid;
+
+int orc_cache_layout(int n_levels, int log2_t, int base_res, double growth, uint32_t* offsets, uint32_t* sizes,
+                     int* res, int* dense, uint32_t* total) {
+  const uint32_t T = 1u << log2_t;
+  uint64_t off = 0;
+  for (int l = 0; l < 16; ++l) {
+    const double n = floor((double)base_res * pow(growth, (double)l));
+    const double verts = (n + 1.0) * (n + 1.0) * (n + 1.0);
+    res[l] = (int)n;
+    dense[l] = verts <= (double)T;
+    sizes[l] = dense[l] ? (uint32_t)(((uint64_t)verts + 7) / 8 * 8) : T;
+    offsets[l] = (uint32_t)off;
+    if (l < n_levels) off += sizes[l];
+  }
+  *total = (uint32_t)off;
+  return 0;
+}
+
+static void orc_corners(const orc_grid* g, int l, const double u[3], uint32_t idx[8], double w[8]) {
+  const int n = g->res[l];
+  int c[3];
+  double f[3];
+  for (int a = 0; a < 3; ++a) {
+    const double s = u[a] * (double)n;
+    int ci = (int)floor(s);
+    ci = ci < 0 ? 0 : (ci > n - 1 ? n - 1 : ci);
+    c[a] = ci;
+    f[a] = s - (double)ci;
+  }
+  const uint32_t stride = (uint32_t)(n + 1);
+  for (int k = 0; k < 8; ++k) {
+    const uint32_t x = (uint32_t)(c[0] + (k & 1)), y = (uint32_t)(c[1] + ((k >> 1) & 1)),
+                   z = (uint32_t)(c[2] + ((k >> 2) & 1));
+    const uint32_t local = g->dense[l] ? x + y * stride + z * stride * stride
+                                       : (x ^ (y * 2654435761u) ^ (z * 805459861u)) & (g->table_size - 1u);
+    idx[k] = g->offset[l] + local;
+    w[k] = ((k & 1) ? f[0] : 1.0 - f[0]) * (((k >> 1) & 1) ? f[1] : 1.0 - f[1]) * (((k >> 2) & 1) ? f[2] : 1.0 - f[2]);
+  }
+}
+
+static void orc_unit(const orc_grid* g, const float* p, double u[3]) {
+  for (int a = 0; a < 3; ++a) {
+    double v = ((double)p[a] - g->in_min[a]) * g->in_scale[a];
+    u[a] = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+  }
+}
+
+static void orc_grid_init(orc_grid* g, int n_levels, int log2_t, int base_res, double growth, const float* in_min,
+                          const float* in_extent) {
+  g->n_levels = n_levels;
+  g->table_size = 1u << log2_t;
+  orc_cache_layout(n_levels, log2_t, base_res, growth, g->offset, g->size, g->res, g->dense, &g->total);
+  for (int a = 0; a < 3; ++a) {
+    g->in_min[a] = in_min[a];
+    g->in_scale[a] = (double)(1.0f / in_extent[a]);
+  }
+}
+
+/* forward of one point: x[32], h[4][64], returns y */
+static double orc_mlp_fwd(const orc_grid* g, const float* grid, const float* wt, const float* p, double x[32],
+                          double h[4][64]) {
+  double u[3];
+  orc_unit(g, p, u);
+  for (int k = 0; k < 32; ++k) x[k] = 0.0;
+  for (int l = 0; l < g->n_levels; ++l) {
+    uint32_t idx[8];
+    double w[8];
+    orc_corners(g, l, u, idx, w);
+    for (int c = 0; c < 8; ++c) {
+      x[2 * l] += w[c] * (double)grid[2 * (size_t)idx[c]];
+      x[2 * l + 1] += w[c] * (double)grid[2 * (size_t)idx[c] + 1];
+    }
+  }
+  const double* in = x;
+  int nin = 32;
+  const int offs[4] = {0, 2048, 6144, 10240};
+  for (int layer = 0; layer < 4; ++layer) {
+    for (int o = 0; o < 64; ++o) {
+      double z = 0.0;
+      for (int k = 0; k < nin; ++k) z += (double)wt[offs[layer] + o * nin + k] * in[k];
+      h[layer][o] = z > 0.0 ? z : 0.0;
+    }
+    in = h[layer];
+    nin = 64;
+  }
+  double y = 0.0;
+  for (int k = 0; k < 64; ++k) y += (double)wt[14336 + k] * h[3][k];
+  return y;
+}
+
+int orc_cache_forward(int n_levels, int log2_t, int base_res, double growth, const float* in_min,
+                      const float* in_extent, const float* grid, const float* wt, const float* pts, long n,
+                      double* out) {
+  orc_grid g;
+  orc_grid_init(&g, n_levels, log2_t, base_res, growth, in_min, in_extent);
+  double x[32], h[4][64];
+  for (long s = 0; s < n; ++s) out[s] = orc_mlp_fwd(&g, grid, wt, pts + 3 * s, x, h);
+  return 0;
+}
+
+/* Gradients of L = (1/n) sum (y - t)^2 w.r.t. the grid and the weights
+ * (accumulated into grid_grad[2 * total] and w_grad[15360], which the caller
+ * zeroes); returns L. */
+double orc_cache_grads(int n_levels, int log2_t, int base_res, double growth, const float* in_min,
+                       const float* in_extent, const float* grid, const float* wt, const float* pts,
+                       const float* targets, long n, double* grid_grad, double* w_grad) {
+  orc_grid g;
+  orc_grid_init(&g, n_levels, log2_t, base_res, growth, in_min, in_extent);
+  const int offs[4] = {0, 2048, 6144, 10240};
+  double loss = 0.0;
+  for (long s = 0; s < n; ++s) {
+    double x[32], h[4][64];
+    const double y = orc_mlp_fwd(&g, grid, wt, pts + 3 * s, x, h);
+    const double diff = y - (double)targets[s];
+    loss += diff * diff;
+    const double dy = 2.0 * diff / (double)n;
+    double dh[64], dz[64];
+    for (int k = 0; k < 64; ++k) {
+      w_grad[14336 + k] += dy * h[3][k];
+      dh[k] = dy * (double)wt[14336 + k];
+    }
+    for (int layer = 3; layer >= 0; --layer) {
+      const int nin = layer == 0 ? 32 : 64;
+      const double* in = layer == 0 ? x : h[layer - 1];
+      for (int o = 0; o < 64; ++o) dz[o] = h[layer][o] > 0.0 ? dh[o] : 0.0;
+      double din[64];
+      for (int k = 0; k < nin; ++k) din[k] = 0.0;
+      for (int o = 0; o < 64; ++o) {
+        for (int k = 0; k < nin; ++k) {
+          w_grad[offs[layer] + o * nin + k] += dz[o] * in[k];
+          din[k] += dz[o] * (double)wt[offs[layer] + o * nin + k];
+        }
+      }
+      for (int k = 0; k < nin; ++k) dh[k] = din[k];
+    }
+    /* dh now holds dL/dx */
+    double u[3];
+    orc_unit(&g, pts + 3 * s, u);
+    for (int l = 0; l < g.n_levels; ++l) {
+      uint32_t idx[8];
+      double w[8];
+      orc_corners(&g, l, u, idx, w);
+      for (int c = 0; c < 8; ++c) {
+        grid_grad[2 * (size_t)idx[c]] += w[c] * dh[2 * l];
+        grid_grad[2 * (size_t)idx[c] + 1] += w[c] * dh[2 * l + 1];
+      }
+    }
+  }
+  return loss / (double)n;
+}

@@ -1,0 +1,105 @@
+"""Learned cache on the GPU against the FP64 oracle: layout, forward (FP16 MFMA
+operands, FP32 accumulation: relative L2 error <= 2e-2), gradients of the
+MSE loss (relative L2 error per parameter group <= 1e-2), Adam's first step in
+closed form, and an overfit test on x^2 - y^2."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a, np.float64) - b) / max(1e-30, np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def small(eng, torch_gpu):
+    from n2wos_amd import cache as K
+    cfg = K.CacheConfig(n_levels=16, log2_table_size=12, base_resolution=4, growth=1.5, seed=3)
+    c = K.Cache(cfg)
+    lay = c.layout()
+    rng = np.random.default_rng(0)
+    grid = rng.normal(0, 0.5, 2 * lay["total_entries"]).astype(np.float32)
+    w = np.zeros(K.N_WEIGHTS, np.float32)
+    w[:2048] = rng.normal(0, 0.25, 2048)
+    w[2048:14336] = rng.normal(0, 0.18, 12288)
+    w[14336:14400] = rng.normal(0, 0.3, 64)
+    c.set_params(grid, w)
+    return cfg, c, grid, w
+
+
+def test_layout_matches_oracle(eng, orc, small):
+    cfg, c, _, _ = small
+    lay = c.layout()
+    ol = orc.cache_layout(cfg.n_levels, cfg.log2_table_size, cfg.base_resolution, cfg.growth)
+    for k in ("offsets", "sizes", "res", "dense", "total_entries"):
+        assert lay[k] == ol[k], k
+
+
+def test_forward_matches_fp64_oracle(eng, orc, torch_gpu, small):
+    torch = torch_gpu
+    cfg, c, grid, w = small
+    pts = np.random.default_rng(1).uniform(-1.1, 1.1, (1000, 3)).astype(np.float32)
+    print("case: cache forward 1000", flush=True)
+    y = c.infer(torch.from_numpy(pts).cuda()).cpu().numpy()
+    ref = orc.cache_forward(cfg, grid, w, pts)
+    assert np.std(ref) > 0.1
+    assert rel(y, ref) < 2e-2
+    assert np.max(np.abs(y - ref)) < 0.1 * np.max(np.abs(ref))
+
+
+def test_gradients_match_fp64_oracle(eng, orc, torch_gpu, small):
+    torch = torch_gpu
+    cfg, c, grid, w = small
+    rng = np.random.default_rng(2)
+    pts = rng.uniform(-1.0, 1.0, (1000, 3)).astype(np.float32)
+    tg = rng.normal(size=1000).astype(np.float32)
+    print("case: cache gradients 1000", flush=True)
+    gg, wg, loss = c.gradients(torch.from_numpy(pts).cuda(), torch.from_numpy(tg).cuda())
+    ogg, owg, oloss = orc.cache_grads(cfg, grid, w, pts, tg)
+    assert abs(loss - oloss) < 2e-2 * oloss
+    for lo, hi in [(0, 2048), (2048, 6144), (6144, 10240), (10240, 14336), (14336, 14400)]:
+        assert rel(wg[lo:hi], owg[lo:hi]) < 1e-2, (lo, hi)
+    assert np.all(wg[14400:] == 0)
+    assert rel(gg, ogg) < 1e-2
+
+
+def test_adam_first_step_closed_form(eng, orc, torch_gpu):
+    """Step 1 of Adam: m_hat = g, v_hat = g^2, so p1 = p0 - lr g / (|g| + eps)."""
+    torch = torch_gpu
+    from n2wos_amd import cache as K
+    cfg = K.CacheConfig(n_levels=4, log2_table_size=10, base_resolution=4, learning_rate=1e-2, seed=5)
+    c = K.Cache(cfg)
+    g0, w0 = c.get_params()
+    rng = np.random.default_rng(3)
+    pts = torch.from_numpy(rng.uniform(-1, 1, (512, 3)).astype(np.float32)).cuda()
+    tg = torch.from_numpy(rng.normal(size=512).astype(np.float32)).cuda()
+    gg, wg, _ = c.gradients(pts, tg)
+    c.train_step(pts, tg)
+    torch.cuda.synchronize()
+    g1, w1 = c.get_params()
+    live = np.abs(wg) > 1e-6
+    expect = w0[live] - 1e-2 * wg[live] / np.abs(wg[live])
+    assert np.allclose(w1[live], expect, atol=2e-4)
+    assert np.array_equal(w1[~live & (wg == 0)], w0[~live & (wg == 0)])
+    liveg = np.abs(gg) > 1e-7
+    assert np.allclose(g1[liveg], g0[liveg] - 1e-2 * np.sign(gg[liveg]), atol=2e-4)
+
+
+def test_cache_overfits_smooth_harmonic(eng, torch_gpu):
+    """Cache probe shape of the reference (fill x^2 - y^2 targets, train,
+    infer, compare): RMSE on held-out points falls below 0.02."""
+    torch = torch_gpu
+    from n2wos_amd import cache as K
+    c = K.Cache(K.CacheConfig(seed=7))
+    g = torch.Generator(device="cuda").manual_seed(0)
+    print("case: cache overfit x^2-y^2", flush=True)
+    for step in range(300):
+        p = torch.rand((1 << 16, 3), device="cuda", generator=g) * 2 - 1
+        t = p[:, 0] ** 2 - p[:, 1] ** 2
+        c.train_step(p.contiguous(), t.contiguous())
+    q = torch.rand((1 << 14, 3), device="cuda", generator=g) * 2 - 1
+    y = c.infer(q.contiguous())
+    rmse = torch.sqrt(torch.mean((y - (q[:, 0] ** 2 - q[:, 1] ** 2)) ** 2)).item()
+    print("overfit rmse", rmse)
+    assert rmse < 0.02

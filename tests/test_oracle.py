@@ -211,3 +211,51 @@ def test_welford_and_chan_against_two_pass(orc):
     for p in range(4):
         seg = xd[250 * p: 250 * (p + 1)]
         assert n[p] == 250 and abs(mean[p] - seg.mean()) < 1e-12 and abs(m2[p] - ((seg - seg.mean()) ** 2).sum()) < 1e-9
+
+
+def test_cache_oracle_gradients_match_finite_differences(orc):
+    """Central differences of the FP64 cache oracle's loss pin its reverse-mode
+    gradients (grid entries and weights of every layer)."""
+    from types import SimpleNamespace
+    cfg = SimpleNamespace(n_levels=3, log2_table_size=8, base_resolution=2, growth=2.0, origin=(-1, -1, -1),
+                          extent=(2, 2, 2))
+    lay = orc.cache_layout(3, 8, 2, 2.0)
+    rng = np.random.default_rng(0)
+    grid = rng.normal(0, 0.5, 2 * lay["total_entries"]).astype(np.float32)
+    w = np.zeros(15360, np.float32)
+    w[:2048] = rng.normal(0, 0.3, 2048)
+    w[2048:14336] = rng.normal(0, 0.2, 12288)
+    w[14336:14400] = rng.normal(0, 0.3, 64)
+    pts = rng.uniform(-0.9, 0.9, (7, 3)).astype(np.float32)
+    tg = rng.normal(size=7).astype(np.float32)
+    gg, wg, loss = orc.cache_grads(cfg, grid, w, pts, tg)
+
+    def L(g, ww):
+        y = orc.cache_forward(cfg, g, ww, pts)
+        return np.mean((y - tg.astype(np.float64)) ** 2)
+
+    assert abs(L(grid, w) - loss) < 1e-12
+    h = 1e-3
+    for k in list(rng.choice(np.nonzero(gg)[0], 6, replace=False)):
+        gp, gm = grid.copy(), grid.copy()
+        gp[k] += h
+        gm[k] -= h
+        fd = (L(gp, w) - L(gm, w)) / (2 * h)
+        assert abs(fd - gg[k]) < 1e-4 * max(1.0, abs(fd))
+    for k in [5, 100, 2048 + 7, 6144 + 300, 10240 + 4000, 14336 + 3]:
+        wp, wm = w.copy(), w.copy()
+        wp[k] += h
+        wm[k] -= h
+        fd = (L(grid, wp) - L(grid, wm)) / (2 * h)
+        assert abs(fd - wg[k]) < 1e-4 * max(1.0, abs(fd))
+
+
+def test_cache_oracle_layout_closed_form(orc):
+    lay = orc.cache_layout(16, 19, 16, 1.5)
+    import math
+    for l in range(16):
+        n = math.floor(16 * 1.5 ** l)
+        assert lay["res"][l] == n
+        assert lay["dense"][l] == int((n + 1) ** 3 <= 2 ** 19)
+    assert lay["dense"][:4] == [1, 1, 1, 1] and lay["dense"][4] == 0
+    assert lay["total_entries"] == sum(lay["sizes"])
