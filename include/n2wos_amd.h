@@ -209,6 +209,30 @@ int n2wos_wos_solve(const n2wos_bvh* bvh, const n2wos_boundary* boundary, const 
                     void* d_workspace, int64_t workspace_bytes, float* d_walk_values, int* d_walk_steps,
                     void* stream);
 
+/* ------------------------------------------------ neural-cache WoS and 2LMC */
+/* Walks of the neural-cache estimators (replacing launch_nc_hybrid_prefix and
+ * launch_nc_2lmc_prefix_continue, include/n2wos/tcnn_nc_wos.hpp:79-134):
+ *   N2WOS_NC_PREFIX:  each walk stops at depth m (or earlier at the boundary);
+ *     d_prefix = accumulated source (+ boundary value if it ended there),
+ *     d_xm = position X_m, d_need = 1 if the cache must be added at X_m;
+ *   N2WOS_NC_COUPLED: each walk records the same depth-m state and continues
+ *     to the boundary; d_full = the full walk's value (shared prefix).
+ * Outputs have n_points * walks_per_point entries (d_xm three floats each);
+ * d_steps receives per-point step totals.  Workspace: n2wos_nc_workspace_size. */
+enum { N2WOS_NC_PREFIX = 1, N2WOS_NC_COUPLED = 2 };
+int64_t n2wos_nc_workspace_size(int n_points, int walks_per_point);
+int n2wos_nc_walks(const n2wos_bvh* bvh, const n2wos_boundary* boundary, const n2wos_walk_params* params,
+                   const float* d_points, int n_points, int walks_per_point, uint32_t point_offset, int mode,
+                   int depth_m, float* d_prefix, float* d_xm, uint8_t* d_need, float* d_full, uint64_t* d_steps,
+                   void* d_workspace, int64_t workspace_bytes, void* stream);
+/* Per-point FP64 statistics of the estimator samples (the reference's combine
+ * step, src/cuda/cubql_bvh.cu:1608-1627): prefix v = a + need C, coupled
+ * v = need (W - a - C).  d_cache holds C(X_m) per walk (entries with need = 0
+ * are ignored).  accumulate != 0 merges into the existing statistics. */
+int n2wos_nc_combine(int n_points, int walks_per_point, int mode, const float* d_prefix, const uint8_t* d_need,
+                     const float* d_cache, const float* d_full, int accumulate, double* d_n, double* d_mean,
+                     double* d_m2, void* stream);
+
 /* ------------------------------------------------------------ learned cache */
 /* Multiresolution hash grid + ReLU MLP (2 n_levels inputs, 4 hidden layers of
  * 64, 1 output), trained with Adam on the MSE loss.  Replaces the

@@ -82,6 +82,9 @@ _SIGNATURES = {
     "n2wos_cache_get_params": ([vp, vp, vp], i32),
     "n2wos_cache_set_params": ([vp, vp, vp], i32),
     "n2wos_cache_gradients": ([vp, vp, vp, i32, vp, vp, vp], i32),
+    "n2wos_nc_workspace_size": ([i32, i32], i64),
+    "n2wos_nc_walks": ([vp, vp, vp, vp, i32, i32, C.c_uint32, i32, i32, vp, vp, vp, vp, vp, vp, i64, vp], i32),
+    "n2wos_nc_combine": ([i32, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp, vp], i32),
     "n2wos_wos_solve": ([vp, vp, vp, vp, i32, i32, C.c_uint32, vp, vp, vp, vp, vp, i32, vp, i64, vp, vp, vp], i32),
 }
 

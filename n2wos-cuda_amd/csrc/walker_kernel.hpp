@@ -54,6 +54,13 @@ __host__ __device__ __forceinline__ void chan_merge(double na, double ma, double
 constexpr int kWaves = N2_WAVES_PER_BLOCK;  // waves per block
 constexpr int kBlock = 64 * kWaves;
 
+// Walk modes: plain walks; prefix walks that stop at depth m and leave the
+// cache query point X_m (neural-cache WoS); coupled walks that record the
+// depth-m state and then continue to the boundary (two-level MC residual).
+constexpr int kModePlain = 0;
+constexpr int kModePrefix = 1;
+constexpr int kModeCoupled = 2;
+
 // Lane phases of the persistent walker.
 constexpr int kIdle = 0;    // no walk: waiting for refill
 constexpr int kNode = 1;    // query in progress, at an inner node
@@ -248,8 +255,24 @@ __global__ void __launch_bounds__(kBlock) k_wos_persistent(WosKernelArgs a) {
         } else {
           const float d = sqrtf(fmaxf(bd2, 0.0f));
           const bool forced = step >= a.max_steps;
-          if (d <= a.epsilon || forced || !isfinite(d)) {
-            const float value = src + boundary_value(a.bc, bcp);
+          const bool at_boundary = d <= a.epsilon || forced || !isfinite(d);
+          // prefix mode ends a walk at depth m (after the boundary test, as the
+          // reference's hybrid prefix does, src/cuda/cubql_bvh.cu:1486-1510)
+          const bool stop_prefix = !at_boundary && a.mode == kModePrefix && step >= a.depth;
+          if (at_boundary || stop_prefix) {
+            const float value = at_boundary ? src + boundary_value(a.bc, bcp) : src;
+            if (a.mode != kModePlain) {
+              const int64_t sidx = static_cast<int64_t>(point) * a.walks_per_point + walk;
+              if (a.mode == kModeCoupled) a.nc_full[sidx] = value;
+              // coupled walks that passed depth m recorded their prefix there
+              if (a.mode == kModePrefix || step <= a.depth) {
+                a.nc_prefix[sidx] = value;
+                a.nc_need[sidx] = stop_prefix ? 1 : 0;
+                a.nc_xm[3 * sidx] = p.x;
+                a.nc_xm[3 * sidx + 1] = p.y;
+                a.nc_xm[3 * sidx + 2] = p.z;
+              }
+            }
             s_val[wv][slot][wiu] = value;
             atomicAdd(&s_steps[wv][slot], static_cast<uint32_t>(step));
             if (forced) atomicAdd(&s_forced[wv][slot], 1u);
@@ -263,6 +286,15 @@ __global__ void __launch_bounds__(kBlock) k_wos_persistent(WosKernelArgs a) {
             unit_done = (old == 1);
             phase = kIdle;
           } else {
+            if (a.mode == kModeCoupled && step == a.depth) {
+              // the coupled walk's NC state at depth m: prefix source and X_m
+              const int64_t sidx = static_cast<int64_t>(point) * a.walks_per_point + walk;
+              a.nc_prefix[sidx] = src;
+              a.nc_need[sidx] = 1;
+              a.nc_xm[3 * sidx] = p.x;
+              a.nc_xm[3 * sidx + 1] = p.y;
+              a.nc_xm[3 * sidx + 2] = p.z;
+            }
             const uint32_t gp = static_cast<uint32_t>(point) + a.point_offset;
             const U4 rr = philox4x32_10(
                 U4{static_cast<uint32_t>(step), static_cast<uint32_t>(walk), gp, 2u * a.stream}, a.seed_lo,

@@ -162,6 +162,12 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
   a.weight_node = tuning_value("weight_node", 10);
   a.weight_tri = tuning_value("weight_tri", 10);
   a.weight_update = tuning_value("weight_update", 10);
+  a.mode = r.mode;
+  a.depth = r.depth;
+  a.nc_prefix = r.nc_prefix;
+  a.nc_xm = r.nc_xm;
+  a.nc_need = r.nc_need;
+  a.nc_full = r.nc_full;
   if (r.ev_begin) N2_HIP(hipEventRecord(static_cast<hipEvent_t>(r.ev_begin), stream));
   if (cap == 16) {
     if (poisson) launch_wos<16, kSlots, true>(a, grid, stream); else launch_wos<16, kSlots, false>(a, grid, stream);
@@ -175,6 +181,39 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
                      a.units_per_point, r.walks_per_point, r.accumulate, a.unit_mean, a.unit_m2, a.unit_steps,
                      a.unit_flags, r.d_n, r.d_mean, r.d_m2, r.d_steps, r.d_forced);
   N2_HIP(hipGetLastError());
+}
+
+// Per point: the estimator samples of prefix (coarse) or coupled (residual)
+// walks combined with the cache values, reduced in walk order (Welford) and
+// merged (Chan) into the point's (n, mean, M2):
+//   prefix:  v = a + (need ? C(X_m) : 0)            (neural-cache estimate)
+//   coupled: v = need ? W - (a + C(X_m)) : 0         (two-level residual)
+// as the reference's combine kernel (src/cuda/cubql_bvh.cu:1608-1627).
+__global__ void k_nc_combine(int n_points, int wpp, int mode, const float* __restrict__ prefix,
+                             const uint8_t* __restrict__ need, const float* __restrict__ cache,
+                             const float* __restrict__ full, int accumulate, double* __restrict__ out_n,
+                             double* __restrict__ out_mean, double* __restrict__ out_m2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_points) return;
+  double m = 0.0, q = 0.0;
+  for (int w = 0; w < wpp; ++w) {
+    const int64_t s = static_cast<int64_t>(i) * wpp + w;
+    const float c = need[s] ? cache[s] : 0.0f;
+    float v;
+    if (mode == kModePrefix)
+      v = prefix[s] + c;
+    else
+      v = need[s] ? full[s] - (prefix[s] + c) : 0.0f;
+    const double x = static_cast<double>(v);
+    const double dlt = x - m;
+    m += dlt / static_cast<double>(w + 1);
+    q += dlt * (x - m);
+  }
+  double N = static_cast<double>(wpp), M = m, Q = q;
+  if (accumulate) chan_merge(out_n[i], out_mean[i], out_m2[i], N, M, Q, N, M, Q);
+  out_n[i] = N;
+  out_mean[i] = M;
+  out_m2[i] = Q;
 }
 
 }  // namespace n2
@@ -246,9 +285,96 @@ extern "C" int n2wos_wos_solve(const n2wos_bvh* bvh, const n2wos_boundary* bound
     r.d_walk_values = d_walk_values;
     r.d_walk_steps = d_walk_steps;
     r.d_visits = static_cast<uint64_t*>(wp->d_visit_counts);
+    r.mode = kModePlain;
+    r.depth = 0;
     r.ev_begin = wp->event_begin;
     r.ev_end = wp->event_end;
     run_wos(bvh, bc, r, static_cast<hipStream_t>(stream));
+    return N2WOS_OK;
+  });
+}
+
+extern "C" int n2wos_nc_walks(const n2wos_bvh* bvh, const n2wos_boundary* boundary, const n2wos_walk_params* wp,
+                              const float* d_points, int n_points, int walks_per_point, uint32_t point_offset,
+                              int mode, int depth_m, float* d_prefix, float* d_xm, uint8_t* d_need, float* d_full,
+                              uint64_t* d_steps, void* d_workspace, int64_t workspace_bytes, void* stream) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(mode == N2WOS_NC_PREFIX || mode == N2WOS_NC_COUPLED, N2WOS_ERR_INVALID, "mode must be prefix or coupled");
+    N2_REQUIRE(depth_m >= 0, N2WOS_ERR_INVALID, "depth_m must be >= 0");
+    N2_REQUIRE(bvh && boundary && wp, N2WOS_ERR_NULL, "NULL handle, boundary or parameters");
+    N2_REQUIRE(n_points >= 0 && walks_per_point >= 1, N2WOS_ERR_INVALID, "bad sizes");
+    if (n_points == 0) return N2WOS_OK;
+    N2_REQUIRE(d_points && d_prefix && d_xm && d_need && d_workspace && d_steps, N2WOS_ERR_NULL,
+               "NULL device pointer");
+    N2_REQUIRE(mode == N2WOS_NC_PREFIX || d_full, N2WOS_ERR_NULL, "coupled mode needs d_full");
+    N2_REQUIRE(workspace_bytes >= wos_workspace_bytes(n_points, walks_per_point) +
+                                      static_cast<int64_t>(n_points) * 28 + 256,
+               N2WOS_ERR_INVALID, "workspace too small (n2wos_nc_workspace_size)");
+    N2_REQUIRE(boundary->n_charges >= 0 && boundary->n_charges <= kMaxCharges, N2WOS_ERR_INVALID,
+               "n_charges must be in [0, 16]");
+    Boundary bc{};
+    bc.mode = boundary->mode;
+    bc.n_charges = boundary->n_charges;
+    for (int i = 0; i < bc.n_charges; ++i)
+      for (int k = 0; k < 4; ++k) bc.charge[i][k] = boundary->charges[i][k];
+    // per-point statistics of the raw walk values go to scratch space after
+    // the walker workspace (the estimator statistics come from n2wos_nc_combine)
+    char* extra = static_cast<char*>(d_workspace) + wos_workspace_bytes(n_points, walks_per_point);
+    extra = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(extra) + 255) & ~uintptr_t{255});
+    WosRun r{};
+    r.d_points = d_points;
+    r.n_points = n_points;
+    r.walks_per_point = walks_per_point;
+    r.point_offset = point_offset;
+    r.max_steps = wp->max_steps;
+    r.epsilon = wp->epsilon;
+    r.step_scale = wp->step_scale;
+    r.seed = wp->seed;
+    r.rng_stream = wp->rng_stream;
+    r.source = wp->source;
+    r.max_blocks = wp->max_blocks;
+    r.d_n = reinterpret_cast<double*>(extra);
+    r.d_mean = r.d_n + n_points;
+    r.d_m2 = r.d_mean + n_points;
+    r.d_steps = d_steps;
+    r.d_forced = reinterpret_cast<uint32_t*>(r.d_m2 + n_points);
+    r.accumulate = 0;
+    r.d_workspace = d_workspace;
+    r.d_walk_values = nullptr;
+    r.d_walk_steps = nullptr;
+    r.d_visits = static_cast<uint64_t*>(wp->d_visit_counts);
+    r.ev_begin = wp->event_begin;
+    r.ev_end = wp->event_end;
+    r.mode = mode == N2WOS_NC_PREFIX ? kModePrefix : kModeCoupled;
+    r.depth = depth_m;
+    r.nc_prefix = d_prefix;
+    r.nc_xm = d_xm;
+    r.nc_need = d_need;
+    r.nc_full = d_full;
+    run_wos(bvh, bc, r, static_cast<hipStream_t>(stream));
+    return N2WOS_OK;
+  });
+}
+
+extern "C" int64_t n2wos_nc_workspace_size(int n_points, int walks_per_point) {
+  if (n_points < 0 || walks_per_point < 1) return N2WOS_ERR_INVALID;
+  return wos_workspace_bytes(n_points, walks_per_point) + static_cast<int64_t>(n_points) * 28 + 256;
+}
+
+extern "C" int n2wos_nc_combine(int n_points, int walks_per_point, int mode, const float* d_prefix,
+                                const uint8_t* d_need, const float* d_cache, const float* d_full, int accumulate,
+                                double* d_n, double* d_mean, double* d_m2, void* stream) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(mode == N2WOS_NC_PREFIX || mode == N2WOS_NC_COUPLED, N2WOS_ERR_INVALID, "mode must be prefix or coupled");
+    N2_REQUIRE(n_points >= 0 && walks_per_point >= 1, N2WOS_ERR_INVALID, "bad sizes");
+    if (n_points == 0) return N2WOS_OK;
+    N2_REQUIRE(d_prefix && d_need && d_cache && d_n && d_mean && d_m2, N2WOS_ERR_NULL, "NULL device pointer");
+    N2_REQUIRE(mode == N2WOS_NC_PREFIX || d_full, N2WOS_ERR_NULL, "coupled mode needs d_full");
+    const int b = 256;
+    hipLaunchKernelGGL(k_nc_combine, dim3((n_points + b - 1) / b), dim3(b), 0, static_cast<hipStream_t>(stream),
+                       n_points, walks_per_point, mode == N2WOS_NC_PREFIX ? kModePrefix : kModeCoupled, d_prefix,
+                       d_need, d_cache, d_full, accumulate, d_n, d_mean, d_m2);
+    N2_HIP(hipGetLastError());
     return N2WOS_OK;
   });
 }

@@ -41,6 +41,12 @@ struct WosKernelArgs {
   int weight_node;        // phase choice: lanes x weight, largest wins
   int weight_tri;
   int weight_update;
+  int mode;               // kModePlain / kModePrefix / kModeCoupled
+  int depth;              // m of the prefix / coupled modes
+  float* nc_prefix;       // per walk: prefix value (or the whole value if the walk ended first)
+  float* nc_xm;           // per walk: X_m (3 floats)
+  uint8_t* nc_need;       // per walk: 1 if the cache must be added at X_m
+  float* nc_full;         // coupled mode: the full walk's value
 };
 
 struct WosRun {
@@ -67,6 +73,12 @@ struct WosRun {
   uint64_t* d_visits;
   void* ev_begin;
   void* ev_end;
+  int mode;
+  int depth;
+  float* nc_prefix;
+  float* nc_xm;
+  uint8_t* nc_need;
+  float* nc_full;
 };
 
 int64_t wos_workspace_bytes(int n_points, int walks_per_point);

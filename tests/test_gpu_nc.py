@@ -1,0 +1,105 @@
+"""Neural-cache walks and the two-level estimator on the GPU: prefix and
+coupled walk outputs bit-exact against the oracle's walks; the combine step
+against numpy in FP64; 2LMC unbiased for an untrained cache (3 sigma against
+the FP64 solution); with the exact solution as the cache the residual has mean
+zero (the reference's oracle-2LMC check, scripts/run_wavefront_oracle_sweep.py:112-160)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.fixture(scope="module")
+def m1(eng, orc, torch_gpu):
+    from n2wos_amd import problems as P
+    v, t = P.make_mesh("mesh1")
+    bvh = eng.Bvh(v, t)
+    return v, t, bvh, orc.Bvh2(v, t)
+
+
+@pytest.mark.parametrize("poisson", [False, True])
+def test_prefix_and_coupled_walks_bit_exact(eng, orc, torch_gpu, m1, poisson):
+    from n2wos_amd import nc as N, problems as P, wos as W
+    torch = torch_gpu
+    v, t, bvh, b2 = m1
+    pts = P.interior_points(bvh, v, 64, seed=4)
+    bc = P.poisson_boundary() if poisson else P.charges_boundary(0)
+    wp = W.WalkParams(epsilon=1e-3, seed=21, source=W.SOURCE_POISSON_SINES if poisson else 0)
+    s = N.NcSolver(bvh, bc, wp)
+    tp = torch.from_numpy(pts).cuda()
+    print("case: nc walks poisson=%s" % poisson, flush=True)
+    for mode, stream in ((N.NC_PREFIX, 1), (N.NC_COUPLED, 2)):
+        b = s.walks(tp, 37, mode, 3, stream)
+        torch.cuda.synchronize()
+        kw = dict(max_steps=256, eps=1e-3, step_scale=0.999, bmode=bc.mode, bparams=bc.oracle_params(),
+                  poisson=poisson, seed=21, stream=stream, nthreads=16)
+        ov, os_, ost, oxm = b2.walks(pts, 37, depth=3, want_xm=True, **kw)
+        need = b.need.cpu().numpy()
+        assert np.array_equal(need, (ost == 2).astype(np.uint8))
+        assert np.array_equal(bits(b.prefix.cpu().numpy()), bits(ov))
+        live = need == 1
+        assert np.array_equal(bits(b.xm.cpu().numpy()[live]), bits(oxm[live]))
+        if mode == N.NC_COUPLED:
+            fv, fs, _, _ = b2.walks(pts, 37, **kw)
+            assert np.array_equal(bits(b.full.cpu().numpy()), bits(fv))
+            assert np.array_equal(b.steps.cpu().numpy(), fs.reshape(64, 37).sum(1))
+
+
+def test_combine_matches_numpy(eng, torch_gpu, m1):
+    from n2wos_amd import nc as N, problems as P
+    torch = torch_gpu
+    v, t, bvh, _ = m1
+    pts = P.interior_points(bvh, v, 50, seed=5)
+    s = N.NcSolver(bvh, P.charges_boundary(0))
+    tp = torch.from_numpy(pts).cuda()
+    b = s.walks(tp, 19, N.NC_COUPLED, 2, 2)
+    cv = torch.randn(50 * 19, device="cuda")
+    st = s.combine(b, cv, 50, 19, N.NC_COUPLED)
+    c = s.combine(b, cv, 50, 19, N.NC_PREFIX)
+    need = b.need.cpu().numpy().astype(bool)
+    pre, full, cvn = b.prefix.cpu().numpy(), b.full.cpu().numpy(), cv.cpu().numpy()
+    r = np.where(need, full - (pre + np.where(need, cvn, 0)), 0).astype(np.float32).astype(np.float64).reshape(50, 19)
+    a = (pre + np.where(need, cvn, 0)).astype(np.float32).astype(np.float64).reshape(50, 19)
+    assert np.allclose(st.mean.cpu().numpy(), r.mean(1), atol=1e-12)
+    assert np.allclose(st.m2.cpu().numpy(), ((r - r.mean(1, keepdims=True)) ** 2).sum(1), atol=1e-10)
+    assert np.allclose(c.mean.cpu().numpy(), a.mean(1), atol=1e-12)
+
+
+def test_two_level_unbiased_with_untrained_cache(eng, torch_gpu, m1):
+    """Whatever the cache, coarse + residual is unbiased: z-scores against the
+    FP64 solution stay standard normal (random-init cache)."""
+    from n2wos_amd import cache as K, nc as N, problems as P, wos as W
+    torch = torch_gpu
+    v, t, bvh, _ = m1
+    pts = P.interior_points(bvh, v, 1024, seed=6)
+    bc = P.charges_boundary(0)
+    s = N.NcSolver(bvh, bc, W.WalkParams(epsilon=1e-4, seed=8))
+    cache = K.Cache(K.CacheConfig.for_bounds(v.min(0), v.max(0), seed=2))
+    print("case: 2lmc unbiased untrained cache", flush=True)
+    est, co, re, _ = s.two_level(cache, torch.from_numpy(pts).cuda(), 256, 256, 2)
+    var = co.m2 / (co.n - 1) / co.n + re.m2 / (re.n - 1) / re.n
+    z = ((est - torch.from_numpy(P.charges_solution_f64(bc.charges, pts)).cuda()) / var.sqrt()).cpu().numpy()
+    assert np.mean(np.abs(z) > 3) < 0.015
+    assert abs(z.mean()) < 5 / np.sqrt(len(z))
+
+
+def test_exact_cache_residual_has_zero_mean(eng, torch_gpu, m1):
+    """With C = the exact solution the residual W - S_m - u(X_m) has mean zero:
+    checks the coupling of prefix and continuation and the stopping rule."""
+    from n2wos_amd import nc as N, problems as P, wos as W
+    torch = torch_gpu
+    v, t, bvh, _ = m1
+    pts = P.interior_points(bvh, v, 512, seed=7)
+    bc = P.charges_boundary(0)
+    s = N.NcSolver(bvh, bc, W.WalkParams(epsilon=1e-4, seed=9))
+    b = s.walks(torch.from_numpy(pts).cuda(), 256, N.NC_COUPLED, 3, 2)
+    exact = torch.from_numpy(P.charges_solution_f64(bc.charges, b.xm.cpu().numpy()).astype(np.float32)).cuda()
+    re = s.combine(b, exact, 512, 256, N.NC_COUPLED)
+    z = (re.mean / (re.m2 / 255 / 256).clamp(min=1e-30).sqrt()).cpu().numpy()
+    z = z[np.isfinite(z) & (re.m2.cpu().numpy() > 0)]
+    assert abs(z.mean()) < 5 / np.sqrt(len(z))
+    assert np.mean(np.abs(z) > 3) < 0.02
