@@ -92,6 +92,20 @@ int n2wos_normalize_to_unit_radius(float* verts, int nv, float* center, float* s
 int64_t n2wos_count_degenerate_triangles(const float* verts, int nv, const uint32_t* tris, int nt,
                                          float area2_eps);
 
+/* Mesh files, replacing load_obj_mesh (src/mesh/obj_loader.cpp:63-107) and
+ * load_ply_mesh (src/mesh/ply_loader.cpp): OBJ, or PLY ascii /
+ * binary_little_endian; polygons fan-triangulated.  kind = "obj", "ply" or
+ * NULL (by file extension).  *verts (3 nv floats) and *tris (3 nt) are
+ * allocated by the library: release them with n2wos_mesh_free. */
+int n2wos_mesh_load(const char* path, const char* kind, float** verts, int* n_verts, uint32_t** tris, int* n_tris);
+int n2wos_mesh_free(float* verts, uint32_t* tris);
+/* Host brute-force closest point (closest_point_bruteforce,
+ * src/geometry/closest_point.cpp:50-68): every triangle, first strict
+ * minimum of d2, the same FP32 arithmetic as the GPU query (bit-identical
+ * results); nthreads <= 0 uses every hardware thread. */
+int n2wos_closest_point_bruteforce(const float* verts, int nv, const uint32_t* tris, int nt, const float* points,
+                                   int n, float* d2, float* closest, int* tri, int nthreads);
+
 /* -------------------------------------------------------------------- BVH */
 /* Builds the BVH on the host and uploads it to the current device.
  * build_method: NULL, "median" or "custom" (median split, leaves of at most
@@ -273,6 +287,80 @@ int n2wos_cache_set_params(n2wos_cache* cache, const float* grid, const float* w
  * (synchronous; for gradient checks).  *loss = mean squared error. */
 int n2wos_cache_gradients(n2wos_cache* cache, const float* d_points, const float* d_targets, int n, float* grid_grad,
                           float* weight_grad, float* loss);
+
+/* ---------------------------------------------------------------- serving
+ * A request server over resident meshes.  The reference has no serving loop;
+ * these entry points have no reference counterpart and serve callers that
+ * evaluate point sets of uneven size against meshes kept on the device.
+ *
+ * A request is a set of interior points of one mesh plus a target RMSE.  It
+ * is admitted when its device state (48 B per point) fits the server's HBM
+ * budget (first come, first served), then runs in rounds: round 0 gives every
+ * point `first_round_walks` walks; each later round gives every point the
+ * walks that its estimated RMSE, sqrt(mean_i var_i / n_i), says are still
+ * missing (x 1.1, in units of 64, at most `max_round_walks`), until the
+ * estimate is <= the target or `max_walks_per_point` is reached.
+ *
+ * Continuous batching: every n2wos_server_step() admits queued requests,
+ * gives each mesh ONE persistent walker launch holding pieces of the current
+ * rounds of all its active requests (max-min fair share of `tick_walks`
+ * walks; whole points only), runs the meshes' launches concurrently on their
+ * own streams with the device's blocks split by work, merges each piece into
+ * its request's per-point FP64 (n, mean, M2), and retires finished requests.
+ * Walk j of request point i uses the Philox counter (step, j, rng_base + i)
+ * under the mesh's seed, and a point's 64-walk units are merged in walk
+ * order, so a request's results are bit-identical whatever else is in the
+ * batch, and equal to n2wos_wos_solve of the same points with point_offset =
+ * rng_base and walks_per_point = the request's final walk count.
+ *
+ * A server and its meshes' BVH handles must be used from one thread at a
+ * time; the BVH handles must outlive the server. */
+typedef struct n2wos_server n2wos_server;
+typedef struct {
+  int64_t hbm_budget_bytes;   /* device bytes for request state (one arena) */
+  int64_t tick_walks;         /* walks per mesh launch per step (>= 64) */
+  int first_round_walks;      /* multiple of 64 */
+  int max_round_walks;        /* multiple of 64 */
+  int max_walks_per_point;    /* multiple of 64 */
+} n2wos_server_config;
+enum { N2WOS_REQ_QUEUED = 0, N2WOS_REQ_ACTIVE = 1, N2WOS_REQ_DONE = 2 };
+typedef struct {
+  int state;                  /* N2WOS_REQ_* */
+  int mesh;
+  int n_points;
+  int rounds;                 /* rounds completed */
+  int walks_per_point;        /* walks merged per point so far */
+  double estimated_rmse;      /* after the last completed round (-1 before) */
+  uint64_t walk_steps;        /* total steps of the finished request (0 before) */
+  double t_submit, t_admit, t_done;  /* seconds since server creation (-1 if not yet) */
+} n2wos_request_info;
+typedef struct {
+  int64_t ticks;
+  int64_t requests_submitted, requests_done;
+  int64_t walks;              /* walks run so far */
+  uint64_t walk_steps;        /* steps of finished requests */
+  int64_t arena_bytes, arena_used_bytes, fixed_bytes;
+  double busy_seconds;        /* wall time inside n2wos_server_step */
+} n2wos_server_stats;
+int n2wos_server_create(const n2wos_server_config* config, n2wos_server** out);
+int n2wos_server_destroy(n2wos_server* server);
+/* Registers a resident mesh with its boundary data and walk parameters
+ * (seed = Philox key of all its requests).  Allocates the mesh's batch
+ * buffers (counted in fixed_bytes) and its stream. */
+int n2wos_server_add_mesh(n2wos_server* server, const n2wos_bvh* bvh, const n2wos_boundary* boundary,
+                          const n2wos_walk_params* params, int* mesh_id);
+/* Queues a request (host points are copied; synchronous).  Fails with
+ * N2WOS_ERR_NOMEM if the request can never fit the budget. */
+int n2wos_server_submit(n2wos_server* server, int mesh_id, const float* host_points, int n_points,
+                        double target_rmse, uint32_t rng_base, int64_t* request_id);
+/* One scheduling step (synchronous).  *n_done = requests finished in it. */
+int n2wos_server_step(n2wos_server* server, int* n_done);
+int n2wos_server_request_info(const n2wos_server* server, int64_t request_id, n2wos_request_info* info);
+/* Copies a finished request's per-point results to host arrays (any may be
+ * NULL) and forgets the request. */
+int n2wos_server_take_result(n2wos_server* server, int64_t request_id, double* n, double* mean, double* m2,
+                             uint64_t* steps, uint32_t* forced);
+int n2wos_server_get_stats(const n2wos_server* server, n2wos_server_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -64,6 +64,9 @@ _SIGNATURES = {
     "n2wos_uv_sphere": ([i32, i32, i32, vp, i32, vp, vp], i32),
     "n2wos_normalize_to_unit_radius": ([vp, i32, vp, vp], i32),
     "n2wos_count_degenerate_triangles": ([vp, i32, vp, i32, f32], i64),
+    "n2wos_mesh_load": ([C.c_char_p, C.c_char_p, C.POINTER(vp), C.POINTER(i32), C.POINTER(vp), C.POINTER(i32)], i32),
+    "n2wos_mesh_free": ([vp, vp], i32),
+    "n2wos_closest_point_bruteforce": ([vp, i32, vp, i32, vp, i32, vp, vp, vp, i32], i32),
     "n2wos_bvh_create": ([vp, i32, vp, i32, i32, C.c_char_p, C.POINTER(vp)], i32),
     "n2wos_bvh_destroy": ([vp], i32),
     "n2wos_bvh_get_stats": ([vp, C.POINTER(BvhStats)], i32),

@@ -97,6 +97,7 @@ __global__ void __launch_bounds__(kBlock) k_wos_persistent(WosKernelArgs a) {
   // walk state
   int phase = kIdle;
   int slot = 0, wiu = 0, point = 0, walk = 0, step = 0;
+  uint32_t ctr_point = 0, ctr_walk = 0;  // Philox counter words of the walk
   V3 p = v3(0.0f, 0.0f, 0.0f);
   float src = 0.0f;
   float thr0 = FLT_MAX;
@@ -214,6 +215,13 @@ __global__ void __launch_bounds__(kBlock) k_wos_persistent(WosKernelArgs a) {
           wiu = fill_next + r;
           point = fill_point;
           walk = fill_w0 + wiu;
+          if (a.point_rng) {
+            ctr_point = a.point_rng[2 * point];
+            ctr_walk = a.point_rng[2 * point + 1] + static_cast<uint32_t>(walk);
+          } else {
+            ctr_point = static_cast<uint32_t>(point) + a.point_offset;
+            ctr_walk = static_cast<uint32_t>(walk);
+          }
           step = 0;
           src = 0.0f;
           thr0 = FLT_MAX;
@@ -295,16 +303,13 @@ __global__ void __launch_bounds__(kBlock) k_wos_persistent(WosKernelArgs a) {
               a.nc_xm[3 * sidx + 1] = p.y;
               a.nc_xm[3 * sidx + 2] = p.z;
             }
-            const uint32_t gp = static_cast<uint32_t>(point) + a.point_offset;
-            const U4 rr = philox4x32_10(
-                U4{static_cast<uint32_t>(step), static_cast<uint32_t>(walk), gp, 2u * a.stream}, a.seed_lo,
-                a.seed_hi);
+            const U4 rr = philox4x32_10(U4{static_cast<uint32_t>(step), ctr_walk, ctr_point, 2u * a.stream},
+                                        a.seed_lo, a.seed_hi);
             const V3 dir = sphere_dir(rr.x, rr.y);
             const float R = a.step_scale * d;
             if (POISSON) {
-              const U4 t = philox4x32_10(
-                  U4{static_cast<uint32_t>(step), static_cast<uint32_t>(walk), gp, 2u * a.stream + 1u}, a.seed_lo,
-                  a.seed_hi);
+              const U4 t = philox4x32_10(U4{static_cast<uint32_t>(step), ctr_walk, ctr_point, 2u * a.stream + 1u},
+                                         a.seed_lo, a.seed_hi);
               const float rho = median3(u01(rr.z), u01(rr.w), u01(t.x));
               const V3 e = sphere_dir(t.y, t.z);
               const float Rr = R * rho;

@@ -97,6 +97,44 @@ int max_blocks_per_cu() {
 
 }  // namespace
 
+Boundary to_boundary(const n2wos_boundary* boundary) {
+  N2_REQUIRE(boundary->n_charges >= 0 && boundary->n_charges <= kMaxCharges, N2WOS_ERR_INVALID,
+             "n_charges must be in [0, 16]");
+  Boundary bc{};
+  bc.mode = boundary->mode;
+  bc.n_charges = boundary->n_charges;
+  for (int i = 0; i < bc.n_charges; ++i)
+    for (int k = 0; k < 4; ++k) bc.charge[i][k] = boundary->charges[i][k];
+  switch (bc.mode) {
+    case kBcX2MinusY2: case kBcZebraK4: case kBcZebraK8: case kBcZebraK12: case kBcConstantOne:
+    case kBcCharges: case kBcPoissonSines: break;
+    default: throw Error(N2WOS_ERR_INVALID, "unknown boundary mode");
+  }
+  return bc;
+}
+
+void check_walk_params(const n2wos_walk_params* wp) {
+  N2_REQUIRE(wp->max_steps >= 0, N2WOS_ERR_INVALID, "max_steps must be >= 0");
+  N2_REQUIRE(wp->epsilon >= 0.0f, N2WOS_ERR_INVALID, "epsilon must be >= 0");
+  N2_REQUIRE(wp->step_scale > 0.0f && wp->step_scale <= 1.0f, N2WOS_ERR_INVALID, "step_scale must be in (0, 1]");
+  N2_REQUIRE(wp->source == N2WOS_SOURCE_NONE || wp->source == N2WOS_SOURCE_POISSON_SINES, N2WOS_ERR_INVALID,
+             "unknown source term");
+}
+
+int wos_full_grid(int source) {
+  int dev = 0, cus = 0;
+  N2_HIP(hipGetDevice(&dev));
+  N2_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  constexpr int kSlots = 8;
+  const bool poisson = source == N2WOS_SOURCE_POISSON_SINES;
+  int per_cu;
+  if (tuning_value("ring", 16) == 16)
+    per_cu = poisson ? max_blocks_per_cu<16, kSlots, true>() : max_blocks_per_cu<16, kSlots, false>();
+  else
+    per_cu = poisson ? max_blocks_per_cu<8, kSlots, true>() : max_blocks_per_cu<8, kSlots, false>();
+  return cus * per_cu;
+}
+
 int64_t wos_workspace_bytes(int n_points, int walks_per_point) {
   const int64_t units = static_cast<int64_t>(n_points) * ((walks_per_point + 63) / 64);
   // counter (256 B pad) + mean + m2 (8 B each) + steps + flags (4 B each)
@@ -122,6 +160,7 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
   a.seed_hi = static_cast<uint32_t>(r.seed >> 32);
   a.stream = r.rng_stream;
   a.point_offset = r.point_offset;
+  a.point_rng = r.d_point_rng;
   a.walk_values = r.d_walk_values;
   a.walk_steps = r.d_walk_steps;
   char* ws = static_cast<char*>(r.d_workspace);
@@ -142,18 +181,10 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
 
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, a.next_unit);
   launch_closest_point(bvh, r.d_points, r.n_points, first_d2, first_cp, first_tri, first_ovf, stream);
-  int dev = 0, cus = 0;
-  N2_HIP(hipGetDevice(&dev));
-  N2_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   constexpr int kSlots = 8;
   const bool poisson = r.source == N2WOS_SOURCE_POISSON_SINES;
   const int cap = tuning_value("ring", 16);
-  int per_cu;
-  if (cap == 16)
-    per_cu = poisson ? max_blocks_per_cu<16, kSlots, true>() : max_blocks_per_cu<16, kSlots, false>();
-  else
-    per_cu = poisson ? max_blocks_per_cu<8, kSlots, true>() : max_blocks_per_cu<8, kSlots, false>();
-  int grid = cus * per_cu;
+  int grid = wos_full_grid(r.source);
   if (r.max_blocks > 0) grid = std::min(grid, r.max_blocks);
   const int64_t needed = (units + kWaves - 1) / kWaves;
   grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(grid, needed)));
@@ -246,23 +277,8 @@ extern "C" int n2wos_wos_solve(const n2wos_bvh* bvh, const n2wos_boundary* bound
                "per-walk outputs must be both set or both NULL");
     N2_REQUIRE(workspace_bytes >= wos_workspace_bytes(n_points, walks_per_point), N2WOS_ERR_INVALID,
                "workspace too small (n2wos_wos_workspace_size)");
-    N2_REQUIRE(wp->max_steps >= 0, N2WOS_ERR_INVALID, "max_steps must be >= 0");
-    N2_REQUIRE(wp->epsilon >= 0.0f, N2WOS_ERR_INVALID, "epsilon must be >= 0");
-    N2_REQUIRE(wp->step_scale > 0.0f && wp->step_scale <= 1.0f, N2WOS_ERR_INVALID, "step_scale must be in (0, 1]");
-    N2_REQUIRE(wp->source == N2WOS_SOURCE_NONE || wp->source == N2WOS_SOURCE_POISSON_SINES, N2WOS_ERR_INVALID,
-               "unknown source term");
-    N2_REQUIRE(boundary->n_charges >= 0 && boundary->n_charges <= kMaxCharges, N2WOS_ERR_INVALID,
-               "n_charges must be in [0, 16]");
-    Boundary bc{};
-    bc.mode = boundary->mode;
-    bc.n_charges = boundary->n_charges;
-    for (int i = 0; i < bc.n_charges; ++i)
-      for (int k = 0; k < 4; ++k) bc.charge[i][k] = boundary->charges[i][k];
-    switch (bc.mode) {
-      case kBcX2MinusY2: case kBcZebraK4: case kBcZebraK8: case kBcZebraK12: case kBcConstantOne:
-      case kBcCharges: case kBcPoissonSines: break;
-      default: throw Error(N2WOS_ERR_INVALID, "unknown boundary mode");
-    }
+    check_walk_params(wp);
+    const Boundary bc = to_boundary(boundary);
     WosRun r{};
     r.d_points = d_points;
     r.n_points = n_points;
@@ -310,13 +326,8 @@ extern "C" int n2wos_nc_walks(const n2wos_bvh* bvh, const n2wos_boundary* bounda
     N2_REQUIRE(workspace_bytes >= wos_workspace_bytes(n_points, walks_per_point) +
                                       static_cast<int64_t>(n_points) * 28 + 256,
                N2WOS_ERR_INVALID, "workspace too small (n2wos_nc_workspace_size)");
-    N2_REQUIRE(boundary->n_charges >= 0 && boundary->n_charges <= kMaxCharges, N2WOS_ERR_INVALID,
-               "n_charges must be in [0, 16]");
-    Boundary bc{};
-    bc.mode = boundary->mode;
-    bc.n_charges = boundary->n_charges;
-    for (int i = 0; i < bc.n_charges; ++i)
-      for (int k = 0; k < 4; ++k) bc.charge[i][k] = boundary->charges[i][k];
+    check_walk_params(wp);
+    const Boundary bc = to_boundary(boundary);
     // per-point statistics of the raw walk values go to scratch space after
     // the walker workspace (the estimator statistics come from n2wos_nc_combine)
     char* extra = static_cast<char*>(d_workspace) + wos_workspace_bytes(n_points, walks_per_point);

@@ -26,6 +26,10 @@ struct WosKernelArgs {
   uint32_t seed_lo, seed_hi;
   uint32_t stream;
   uint32_t point_offset;
+  // optional, 2 per point: Philox counter point id and walk-index base (used
+  // instead of point + point_offset and walk, so a point's walks are the
+  // same whichever batch it runs in)
+  const uint32_t* point_rng;
   int* next_unit;
   double* unit_mean;
   double* unit_m2;
@@ -54,6 +58,7 @@ struct WosRun {
   int n_points;
   int walks_per_point;
   uint32_t point_offset;
+  const uint32_t* d_point_rng;
   int max_steps;
   float epsilon;
   float step_scale;
@@ -82,6 +87,11 @@ struct WosRun {
 };
 
 int64_t wos_workspace_bytes(int n_points, int walks_per_point);
+// Validated conversions of the ABI structs (throw Error on bad values).
+Boundary to_boundary(const n2wos_boundary* boundary);
+void check_walk_params(const n2wos_walk_params* wp);
+// Blocks of a full-device walker launch (CUs x resident blocks per CU).
+int wos_full_grid(int source);
 // Enqueues the stand-alone closest-point kernel (bvh.hip).
 void launch_closest_point(const n2wos_bvh* bvh, const float* d_points, int n, float* d_d2, float* d_cp, int* d_tri,
                           int* d_ovf, hipStream_t stream);

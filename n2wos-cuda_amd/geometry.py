@@ -46,6 +46,33 @@ def count_degenerate_triangles(verts, tris, area2_eps: float = 1e-20) -> int:
     return int(lib().n2wos_count_degenerate_triangles(_ptr(v), len(v), _ptr(t), len(t), area2_eps))
 
 
+def load_mesh(path: str, kind: str | None = None):
+    """(vertices (n, 3) float32, triangles (m, 3) uint32) of an OBJ or PLY file."""
+    v, t = C.c_void_p(), C.c_void_p()
+    nv, nt = C.c_int(), C.c_int()
+    check(lib().n2wos_mesh_load(path.encode(), kind.encode() if kind else None, C.byref(v), C.byref(nv),
+                                C.byref(t), C.byref(nt)))
+    try:
+        verts = np.ctypeslib.as_array(C.cast(v, C.POINTER(C.c_float)), (nv.value, 3)).copy()
+        tris = np.ctypeslib.as_array(C.cast(t, C.POINTER(C.c_uint32)), (nt.value, 3)).copy()
+    finally:
+        lib().n2wos_mesh_free(v, t)
+    return verts, tris
+
+
+def closest_point_bruteforce(verts, tris, points, nthreads: int = 0):
+    """Host brute force over every triangle: (d2, closest (n, 3), triangle id)."""
+    v = np.ascontiguousarray(verts, np.float32)
+    t = np.ascontiguousarray(tris, np.uint32)
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    d2 = np.empty(len(p), np.float32)
+    cp = np.empty((len(p), 3), np.float32)
+    tri = np.empty(len(p), np.int32)
+    check(lib().n2wos_closest_point_bruteforce(_ptr(v), len(v), _ptr(t), len(t), _ptr(p), len(p), _ptr(d2),
+                                               _ptr(cp), _ptr(tri), nthreads))
+    return d2, cp, tri
+
+
 def build_host(verts, tris, leaf_size: int = 8):
     """Node and triangle-record arrays exactly as uploaded (no device needed)."""
     v = np.ascontiguousarray(verts, np.float32)

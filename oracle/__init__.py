@@ -37,9 +37,9 @@ _ref = None
 
 
 def build(force: bool = False) -> None:
-    """Compile the oracle (and the reference core when its tree is present)."""
-    if force or not os.path.exists(LIB_PATH):
-        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    """Compile the oracle (and the reference core when its tree is present);
+    make rebuilds only what is older than its sources unless force is set."""
+    subprocess.run(["make", "-s", "-C", HERE] + (["-B"] if force else []), check=True, stdout=subprocess.DEVNULL)
 
 
 def lib():
@@ -91,6 +91,11 @@ def lib():
         L.orc_cache_grads.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _f32p, _f32p, _f32p, _f32p, _f32p,
                                       _f32p, C.c_long, _f64p, _f64p]
         L.orc_cache_grads.restype = C.c_double
+        L.orc_round_half.argtypes = [C.c_double]
+        L.orc_round_half.restype = C.c_double
+        L.orc_cache_forward_fp16.argtypes = L.orc_cache_forward.argtypes
+        L.orc_cache_grads_fp16.argtypes = L.orc_cache_grads.argtypes + [C.c_double]
+        L.orc_cache_grads_fp16.restype = C.c_double
         _lib = L
     return _lib
 
@@ -279,11 +284,14 @@ def _cfg_args(cfg):
             _f32(cfg.origin), _f32(cfg.extent))
 
 
-def cache_forward(cfg, grid, weights, pts):
-    """FP64 forward of the cache at (n, 3) points (cfg: an n2wos_amd CacheConfig)."""
+def cache_forward(cfg, grid, weights, pts, fp16=False):
+    """FP64 forward of the cache at (n, 3) points (cfg: an n2wos_amd CacheConfig).
+    fp16=True rounds weights, features and activations to binary16 where the
+    product's MFMA kernels do."""
     pts = _f32(pts).reshape(-1, 3)
     out = np.zeros(len(pts))
-    lib().orc_cache_forward(*_cfg_args(cfg), _f32(grid), _f32(weights), pts, len(pts), out)
+    fn = lib().orc_cache_forward_fp16 if fp16 else lib().orc_cache_forward
+    fn(*_cfg_args(cfg), _f32(grid), _f32(weights), pts, len(pts), out)
     return out
 
 
@@ -293,4 +301,15 @@ def cache_grads(cfg, grid, weights, pts, targets):
     gg = np.zeros(len(grid))
     wg = np.zeros(len(weights))
     loss = lib().orc_cache_grads(*_cfg_args(cfg), _f32(grid), _f32(weights), pts, _f32(targets), len(pts), gg, wg)
+    return gg, wg, loss
+
+
+def cache_grads_fp16(cfg, grid, weights, pts, targets, loss_scale):
+    """cache_grads with the product's binary16 rounding points (weights,
+    features, activations, loss-scaled dL/dy and pre-activation gradients)."""
+    pts = _f32(pts).reshape(-1, 3)
+    gg = np.zeros(len(grid))
+    wg = np.zeros(len(weights))
+    loss = lib().orc_cache_grads_fp16(*_cfg_args(cfg), _f32(grid), _f32(weights), pts, _f32(targets), len(pts), gg,
+                                      wg, float(loss_scale))
     return gg, wg, loss

@@ -1161,9 +1161,27 @@ static void orc_grid_init(orc_grid* g, int n_levels, int log2_t, int base_res, d
   }
 }
 
-/* forward of one point: x[32], h[4][64], returns y */
-static double orc_mlp_fwd(const orc_grid* g, const float* grid, const float* wt, const float* p, double x[32],
-                          double h[4][64]) {
+/* Round to the nearest IEEE binary16 value (ties to even; subnormals kept,
+ * overflow to infinity), returned as a double.  Used by the FP16-rounding
+ * variant of the cache oracle. */
+double orc_round_half(double v) {
+  if (v == 0.0 || !isfinite(v)) return v;
+  const double a = fabs(v);
+  if (a >= 65520.0) return copysign(INFINITY, v);
+  int e;
+  frexp(a, &e);           /* a in [2^(e-1), 2^e) */
+  int ex = e - 1;
+  if (ex < -14) ex = -14; /* subnormal range: fixed spacing 2^-24 */
+  const double q = ldexp(1.0, ex - 10);
+  return copysign(nearbyint(a / q) * q, v);
+}
+
+/* forward of one point: x[32], h[4][64], returns y.  With half != 0 the
+ * weights, the encoded features and every hidden activation are rounded to
+ * binary16 where the product's kernels round them (FP16 MFMA operands); sums
+ * stay in double. */
+static double orc_mlp_fwd_ex(const orc_grid* g, const float* grid, const float* wt, const float* p, double x[32],
+                             double h[4][64], int half) {
   double u[3];
   orc_unit(g, p, u);
   for (int k = 0; k < 32; ++k) x[k] = 0.0;
@@ -1176,21 +1194,33 @@ static double orc_mlp_fwd(const orc_grid* g, const float* grid, const float* wt,
       x[2 * l + 1] += w[c] * (double)grid[2 * (size_t)idx[c] + 1];
     }
   }
+  if (half)
+    for (int k = 0; k < 32; ++k) x[k] = orc_round_half(x[k]);
   const double* in = x;
   int nin = 32;
   const int offs[4] = {0, 2048, 6144, 10240};
   for (int layer = 0; layer < 4; ++layer) {
     for (int o = 0; o < 64; ++o) {
       double z = 0.0;
-      for (int k = 0; k < nin; ++k) z += (double)wt[offs[layer] + o * nin + k] * in[k];
-      h[layer][o] = z > 0.0 ? z : 0.0;
+      for (int k = 0; k < nin; ++k) {
+        const double wk = half ? orc_round_half((double)wt[offs[layer] + o * nin + k]) : (double)wt[offs[layer] + o * nin + k];
+        z += wk * in[k];
+      }
+      const double r = z > 0.0 ? z : 0.0;
+      h[layer][o] = half ? orc_round_half(r) : r;
     }
     in = h[layer];
     nin = 64;
   }
   double y = 0.0;
-  for (int k = 0; k < 64; ++k) y += (double)wt[14336 + k] * h[3][k];
+  for (int k = 0; k < 64; ++k)
+    y += (half ? orc_round_half((double)wt[14336 + k]) : (double)wt[14336 + k]) * h[3][k];
   return y;
+}
+
+static double orc_mlp_fwd(const orc_grid* g, const float* grid, const float* wt, const float* p, double x[32],
+                          double h[4][64]) {
+  return orc_mlp_fwd_ex(g, grid, wt, p, x, h, 0);
 }
 
 int orc_cache_forward(int n_levels, int log2_t, int base_res, double growth, const float* in_min,
@@ -1206,39 +1236,53 @@ int orc_cache_forward(int n_levels, int log2_t, int base_res, double growth, con
 /* Gradients of L = (1/n) sum (y - t)^2 w.r.t. the grid and the weights
  * (accumulated into grid_grad[2 * total] and w_grad[15360], which the caller
  * zeroes); returns L. */
-double orc_cache_grads(int n_levels, int log2_t, int base_res, double growth, const float* in_min,
-                       const float* in_extent, const float* grid, const float* wt, const float* pts,
-                       const float* targets, long n, double* grid_grad, double* w_grad) {
+static double orc_cache_grads_ex(int n_levels, int log2_t, int base_res, double growth, const float* in_min,
+                                 const float* in_extent, const float* grid, const float* wt, const float* pts,
+                                 const float* targets, long n, double* grid_grad, double* w_grad, double loss_scale) {
+  /* loss_scale > 0 selects the FP16-rounding variant: dL/dy is scaled by
+   * loss_scale, it and every pre-activation gradient dZ are rounded to
+   * binary16 (as the product stores them for its MFMA weight-gradient GEMMs),
+   * and the sums are unscaled at the end; the input-feature gradient is the
+   * unrounded sum (an FP32 accumulator in the product). */
+  const int half = loss_scale > 0.0;
+  const double scale = half ? loss_scale : 1.0;
   orc_grid g;
   orc_grid_init(&g, n_levels, log2_t, base_res, growth, in_min, in_extent);
   const int offs[4] = {0, 2048, 6144, 10240};
   double loss = 0.0;
   for (long s = 0; s < n; ++s) {
     double x[32], h[4][64];
-    const double y = orc_mlp_fwd(&g, grid, wt, pts + 3 * s, x, h);
+    const double y = orc_mlp_fwd_ex(&g, grid, wt, pts + 3 * s, x, h, half);
     const double diff = y - (double)targets[s];
     loss += diff * diff;
-    const double dy = 2.0 * diff / (double)n;
+    double dy = 2.0 * diff * (scale / (double)n);
+    if (half) dy = orc_round_half(dy);
     double dh[64], dz[64];
     for (int k = 0; k < 64; ++k) {
-      w_grad[14336 + k] += dy * h[3][k];
-      dh[k] = dy * (double)wt[14336 + k];
+      const double w5 = half ? orc_round_half((double)wt[14336 + k]) : (double)wt[14336 + k];
+      w_grad[14336 + k] += dy * h[3][k] / scale;
+      dh[k] = dy * w5;
     }
     for (int layer = 3; layer >= 0; --layer) {
       const int nin = layer == 0 ? 32 : 64;
       const double* in = layer == 0 ? x : h[layer - 1];
-      for (int o = 0; o < 64; ++o) dz[o] = h[layer][o] > 0.0 ? dh[o] : 0.0;
+      for (int o = 0; o < 64; ++o) {
+        const double d = h[layer][o] > 0.0 ? dh[o] : 0.0;
+        dz[o] = half ? orc_round_half(d) : d;
+      }
       double din[64];
       for (int k = 0; k < nin; ++k) din[k] = 0.0;
       for (int o = 0; o < 64; ++o) {
         for (int k = 0; k < nin; ++k) {
-          w_grad[offs[layer] + o * nin + k] += dz[o] * in[k];
-          din[k] += dz[o] * (double)wt[offs[layer] + o * nin + k];
+          const double wk = half ? orc_round_half((double)wt[offs[layer] + o * nin + k])
+                                 : (double)wt[offs[layer] + o * nin + k];
+          w_grad[offs[layer] + o * nin + k] += dz[o] * in[k] / scale;
+          din[k] += dz[o] * wk;
         }
       }
       for (int k = 0; k < nin; ++k) dh[k] = din[k];
     }
-    /* dh now holds dL/dx */
+    /* dh now holds dL/dx (times the scale) */
     double u[3];
     orc_unit(&g, pts + 3 * s, u);
     for (int l = 0; l < g.n_levels; ++l) {
@@ -1246,10 +1290,36 @@ double orc_cache_grads(int n_levels, int log2_t, int base_res, double growth, co
       double w[8];
       orc_corners(&g, l, u, idx, w);
       for (int c = 0; c < 8; ++c) {
-        grid_grad[2 * (size_t)idx[c]] += w[c] * dh[2 * l];
-        grid_grad[2 * (size_t)idx[c] + 1] += w[c] * dh[2 * l + 1];
+        grid_grad[2 * (size_t)idx[c]] += w[c] * dh[2 * l] / scale;
+        grid_grad[2 * (size_t)idx[c] + 1] += w[c] * dh[2 * l + 1] / scale;
       }
     }
   }
   return loss / (double)n;
+}
+
+double orc_cache_grads(int n_levels, int log2_t, int base_res, double growth, const float* in_min,
+                       const float* in_extent, const float* grid, const float* wt, const float* pts,
+                       const float* targets, long n, double* grid_grad, double* w_grad) {
+  return orc_cache_grads_ex(n_levels, log2_t, base_res, growth, in_min, in_extent, grid, wt, pts, targets, n,
+                            grid_grad, w_grad, 0.0);
+}
+
+/* The same gradients with the product's binary16 rounding points (weights,
+ * features, activations, scaled dL/dy and dZ); loss_scale as the product's. */
+double orc_cache_grads_fp16(int n_levels, int log2_t, int base_res, double growth, const float* in_min,
+                            const float* in_extent, const float* grid, const float* wt, const float* pts,
+                            const float* targets, long n, double* grid_grad, double* w_grad, double loss_scale) {
+  return orc_cache_grads_ex(n_levels, log2_t, base_res, growth, in_min, in_extent, grid, wt, pts, targets, n,
+                            grid_grad, w_grad, loss_scale);
+}
+
+int orc_cache_forward_fp16(int n_levels, int log2_t, int base_res, double growth, const float* in_min,
+                           const float* in_extent, const float* grid, const float* wt, const float* pts, long n,
+                           double* out) {
+  orc_grid g;
+  orc_grid_init(&g, n_levels, log2_t, base_res, growth, in_min, in_extent);
+  double x[32], h[4][64];
+  for (long s = 0; s < n; ++s) out[s] = orc_mlp_fwd_ex(&g, grid, wt, pts + 3 * s, x, h, 1);
+  return 0;
 }

@@ -1,0 +1,37 @@
+"""The probe CLI on the GPU: benchmark problem 1 (UV sphere 64 x 128,
+r = 1 + 0.1 sin 6t sin 6p, normalised; 65,536 queries uniform in
+[-1.2, 1.2]^3) exits 0, writes the n2wos_cuda_probe_bvh_v1 keys, and every
+validated query equals the host brute force bit for bit."""
+import json
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KEYS = {"schema", "patch", "generated_at_utc", "command_line", "cuda", "implementation_mode", "options", "mesh_stats",
+        "bvh_stats", "timing", "validation"}
+MODE_KEYS = {"gpu_resident_geometry_query", "geometry_query", "fcpw_gpu_public_api", "host_driven_walker_loop",
+             "cache_inference", "rng", "accumulation", "timing_scope"}
+
+
+def test_probe_cli_problem1(eng, torch_gpu, tmp_path):
+    exe = os.path.join(os.path.dirname(eng.LIB_PATH), "..", "bin", "n2wos_probe_cuda_bvh")
+    out = str(tmp_path / "probe.json")
+    args = [exe, "--mesh", "sin6", "--bumpy-stacks", "64", "--bumpy-slices", "128", "--bumpy-amplitude", "0.1",
+            "--queries", "65536", "--query-extent", "1.2", "--seed", "0", "--validate", "4096", "--repeat", "3",
+            "--output", out]
+    print("case: probe CLI problem 1", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    j = json.load(open(out))
+    assert KEYS <= set(j) and MODE_KEYS <= set(j["implementation_mode"])
+    assert j["schema"] == "n2wos_cuda_probe_bvh_v1"
+    assert j["implementation_mode"]["geometry_query"] == "hip_bvh_gfx950"
+    assert j["mesh_stats"]["triangles"] == 16128 and j["mesh_stats"]["vertices"] == 8066
+    v = j["validation"]
+    assert v["checked"] == 4096 and v["bad_distance_count_threshold_2e_4"] == 0
+    assert v["bit_exact_matches"] == 4096 and v["triangle_id_mismatches"] == 0
+    assert v["cuda_stack_overflow_count"] == 0
+    assert len(j["timing"]["kernel_ms"]) == 3 and j["timing"]["median_mqueries_per_second"] > 0

@@ -1,0 +1,103 @@
+"""Request server: a request's results are bit-identical alone and in a mix of
+requests on three resident meshes, equal the direct solve of the same walks,
+meet the target RMSE against the FP64 solution, and admission follows the
+HBM budget in arrival order."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def meshes(eng, torch_gpu):
+    from n2wos_amd import problems as P, wos as W
+    out = []
+    for k, name in enumerate(("mesh1", "mesh2", "blob")):
+        v, t = P.make_mesh(name)
+        bvh = eng.Bvh(v, t, leaf_size=4)
+        bc = P.charges_boundary(k)
+        wp = W.WalkParams(max_steps=256, epsilon=1e-4, seed=900 + k)
+        out.append((name, v, bvh, bc, wp))
+    return out
+
+
+def make_server(eng, meshes, **cfg):
+    from n2wos_amd import serve as S
+    s = S.Server(S.ServerConfig(**cfg))
+    for _, _, bvh, bc, wp in meshes:
+        s.add_mesh(bvh, bc, wp)
+    return s
+
+
+def same(a, b):
+    return (np.array_equal(a.n, b.n) and np.array_equal(a.mean.view(np.uint64), b.mean.view(np.uint64))
+            and np.array_equal(a.m2.view(np.uint64), b.m2.view(np.uint64)) and np.array_equal(a.steps, b.steps)
+            and np.array_equal(a.forced, b.forced))
+
+
+def test_alone_equals_mixed_and_direct(eng, torch_gpu, meshes):
+    from n2wos_amd import problems as P, wos as W
+    torch = torch_gpu
+    sets = [P.interior_points(meshes[m][2], meshes[m][1], n, seed=50 + i)
+            for i, (m, n) in enumerate([(0, 1500), (1, 3000), (2, 2000), (0, 700), (2, 400)])]
+    mesh_of = [0, 1, 2, 0, 2]
+    bases = [0, 10_000, 20_000, 30_000, 40_000]
+    target = 2e-2
+
+    print("case: serve request 0 alone", flush=True)
+    s1 = make_server(eng, meshes, tick_walks=1 << 20)
+    a = s1.submit(0, sets[0], target, rng_base=bases[0])
+    s1.drain([a])
+    info_alone = s1.info(a)
+    alone = s1.take_result(a)
+    assert info_alone.rounds >= 1 and info_alone.estimated_rmse <= target
+
+    print("case: serve five requests mixed, small steps", flush=True)
+    s2 = make_server(eng, meshes, tick_walks=1 << 16)
+    ids = [s2.submit(mesh_of[i], sets[i], target, rng_base=bases[i]) for i in range(5)]
+    s2.drain(ids)
+    assert s2.info(ids[0]).walks_per_point == info_alone.walks_per_point
+    mixed = [s2.take_result(i) for i in ids]
+    assert same(alone, mixed[0])
+    st = s2.stats()
+    assert st["requests_done"] == 5 and st["arena_used_bytes"] == 0
+    assert st["ticks"] > 5  # pieces really were split over many steps
+
+    print("case: served request vs direct solve", flush=True)
+    _, _, bvh, bc, wp = meshes[0]
+    W_ = info_alone.walks_per_point
+    sol = W.Solver(bvh, bc, wp).solve(torch.from_numpy(sets[0]).cuda(), W_, point_offset=bases[0])
+    torch.cuda.synchronize()
+    assert np.array_equal(sol.mean.cpu().numpy().view(np.uint64), alone.mean.view(np.uint64))
+    assert np.array_equal(sol.m2.cpu().numpy().view(np.uint64), alone.m2.view(np.uint64))
+    assert np.array_equal(sol.steps.cpu().numpy(), alone.steps)
+
+    print("case: served accuracy vs FP64 solution", flush=True)
+    for i in range(5):
+        truth = P.charges_solution_f64(meshes[mesh_of[i]][3].charges, sets[i])
+        r = mixed[i]
+        rmse = float(np.sqrt(np.mean((r.mean - truth) ** 2)))
+        est = float(np.sqrt(np.mean(r.m2 / (r.n - 1) / r.n)))
+        assert est <= target * 1.0000001
+        # the estimate is an unbiased variance estimate: the true error agrees
+        assert rmse <= 1.25 * target, (i, rmse, est)
+
+
+def test_budget_admission_order(eng, torch_gpu, meshes):
+    from n2wos_amd import _lib, problems as P, serve as S
+    pts = [P.interior_points(meshes[1][2], meshes[1][1], 4000, seed=70 + i) for i in range(3)]
+    per = 4000 * 48 + 6 * 256
+    print("case: budget admits one request at a time", flush=True)
+    s = make_server(eng, meshes, hbm_budget_bytes=per + 1024, tick_walks=1 << 18)
+    ids = [s.submit(1, p, 3e-2) for p in pts]
+    s.step()
+    states = [s.info(i).state for i in ids]
+    assert states[0] == S.REQ_ACTIVE or states[0] == S.REQ_DONE
+    assert states[1] == S.REQ_QUEUED and states[2] == S.REQ_QUEUED
+    s.drain(ids)
+    inf = [s.info(i) for i in ids]
+    assert inf[0].t_done <= inf[1].t_admit and inf[1].t_done <= inf[2].t_admit
+    with pytest.raises(_lib.N2wosError) as e:
+        s.submit(1, np.zeros((8000, 3), np.float32), 1e-2)
+    assert e.value.code == _lib.ERR_NOMEM
+    assert s.stats()["arena_used_bytes"] == 0

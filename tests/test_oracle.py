@@ -259,3 +259,23 @@ def test_cache_oracle_layout_closed_form(orc):
         assert lay["dense"][l] == int((n + 1) ** 3 <= 2 ** 19)
     assert lay["dense"][:4] == [1, 1, 1, 1] and lay["dense"][4] == 0
     assert lay["total_entries"] == sum(lay["sizes"])
+
+
+def test_half_rounding_matches_numpy_float16(orc):
+    """The oracle's binary16 rounding (used by its FP16-rounding cache variant)
+    equals numpy's correctly rounded float64 -> float16 conversion: normal
+    and subnormal ranges, exact ties (to even), overflow and signed zero."""
+    L = orc.lib()
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=60000) * np.exp(rng.uniform(-22, 12, 60000))
+    half = np.float16
+    # exact ties between neighbouring binary16 values, normal and subnormal
+    a = rng.uniform(-60000, 60000, 2000).astype(half).astype(np.float64)
+    b = np.nextafter(a.astype(half), half(np.inf)).astype(np.float64)
+    s = (rng.integers(0, 1024, 2000) * 2.0 ** -24).astype(half).astype(np.float64)
+    ties = np.concatenate([(a + b) / 2, s + 2.0 ** -25, [65519.0, 65520.0, -65520.0, 2.0 ** -25, 3 * 2.0 ** -25]])
+    vals = np.concatenate([x, ties, [0.0, -0.0, 1e-30, -1e-30]])
+    got = np.array([L.orc_round_half(float(v)) for v in vals])
+    want = vals.astype(half).astype(np.float64)
+    assert np.array_equal(np.signbit(got[want == 0]), np.signbit(want[want == 0]))
+    assert np.array_equal(got, want)
