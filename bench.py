@@ -16,8 +16,26 @@ records x 40 B + 12 B per walk start + 24 B per work unit, visit counts from a
 counting run with the same seeds) against MI355X HBM peak, with the kernel's
 duration measured by HIP events on its own stream.  `cpu_baseline` times the
 oracle's CPU walker (same walks, same Philox streams) on a sample of the
-points.  Extra figures (closest-point queries/s, RMSE against the FP64
-solution) sit under "extra".
+points.  Extra figures sit under "extra":
+  closest_point_problem1   65,536 uniform queries on the 64 x 128 sphere
+  closest_point_stress     262,144 queries each from uniform box, near-
+                           boundary shell (|d| < 0.02) and WoS prefixes (1-20
+                           steps) on the 128 x 256 sphere: queries/s and HBM
+                           roofline fraction of the query kernel
+  wos_problem1             4,096 interior points x 128 walks, eps 1e-3, on the
+                           64 x 128 sphere; the oracle's CPU walker on the
+                           same walks in full beside it
+  poisson_problem3         262,144 interior points x 512 walks on the
+                           512 x 1024 multi-frequency sphere (1,046,528 tris),
+                           Green's-function source sampling, RMSE vs FP64 u*
+  time_to_rmse             plain WoS vs neural cache (NC) and NC + 2LMC to
+                           RMSE <= 1e-2 on the headline problem, cache
+                           training included in the NC times
+  cache                    hash-grid + MLP training step and inference rates
+  serving                  request server: three resident meshes, seeded
+                           Poisson arrivals of log-uniform request sizes,
+                           p50 / p99 latency and throughput
+--quick skips the heavy sections (problem 3, time-to-RMSE, serving).
 """
 from __future__ import annotations
 
@@ -26,6 +44,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -45,6 +65,9 @@ def parse():
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="skip problem 3, time-to-RMSE and serving")
+    ap.add_argument("--serve-requests", type=int, default=12)
+    ap.add_argument("--serve-rate", type=float, default=0.3, help="request arrivals per second")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run a 1 GiB device copy (counter calibration) and one walker solve, then exit; "
                          "used under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
@@ -59,13 +82,143 @@ def cpu_threads() -> int:
     return max(1, min(16, n))
 
 
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def timed_events(torch, fn, reps: int) -> float:
+    """Average ms of fn() over reps launches on torch's current stream."""
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def closest_point_stress(eng, torch, P, n: int = 262144) -> dict:
+    """Queries/s and the query kernel's HBM roofline fraction per distribution.
+    Algorithmic bytes per query: 12 (point) + 20 (d2, closest, id) + 112 per
+    4-wide node loaded + 40 per triangle record tested, counts from the
+    counting build of the same traversal."""
+    v, t = P.make_mesh("mesh2")
+    bvh = eng.Bvh(v, t)
+    interior = P.interior_points(bvh, v, 65536, seed=3)
+    sets = {
+        "uniform_box": P.uniform_box_queries(n, seed=11),
+        "near_boundary_shell": P.surface_shell_queries(v, t, n, seed=12),
+        "wos_prefixes": P.wos_prefix_queries(bvh, interior, n, seed=13),
+    }
+    out = {}
+    for name, q in sets.items():
+        qt = torch.from_numpy(q).cuda()
+        bvh.query_tensors(qt)
+        ms = timed_events(torch, lambda: bvh.query_tensors(qt), 10)
+        _, _, _, ovf, nn, nt = bvh.query_tensors(qt, counted=True)
+        torch.cuda.synchronize()
+        nodes, tris = int(nn.sum().item()), int(nt.sum().item())
+        byts = 32 * n + NODE_BYTES * nodes + TRI_BYTES * tris
+        gbs = byts / (ms * 1e-3) / 1e9
+        out[name] = {"queries": n, "kernel_ms": ms, "queries_per_s": n / (ms * 1e-3),
+                     "node_lines_per_query": nodes / n, "triangle_tests_per_query": tris / n,
+                     "algorithmic_GBps": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
+                     "overflow": int(ovf.sum().item())}
+        log(f"closest point {name}: {n / (ms * 1e-3) / 1e6:.1f} M q/s, {gbs:.0f} GB/s")
+    return out
+
+
+def wos_problem1(eng, torch, P, W, cpu: bool, threads: int) -> dict:
+    v, t = P.make_mesh("mesh1")
+    bvh = eng.Bvh(v, t)
+    pts = P.interior_points(bvh, v, 4096, seed=0)
+    bc = P.charges_boundary(0)
+    wp = W.WalkParams(max_steps=256, epsilon=1e-3, step_scale=0.999, seed=7)
+    solver = W.Solver(bvh, bc, wp)
+    tp = torch.from_numpy(pts).cuda()
+    solver.solve(tp, 128, seed=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = solver.solve(tp, 128, seed=wp.seed)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    steps = int(st.steps.sum().item())
+    truth = P.charges_solution_f64(bc.charges, pts)
+    mean = st.mean.cpu().numpy()
+    out = {"points": 4096, "walks_per_point": 128, "seconds": dt, "walk_steps_per_s": steps / dt,
+           "rmse_vs_fp64": float(np.sqrt(np.mean((mean - truth) ** 2))),
+           "mean_steps_per_walk": steps / (4096 * 128)}
+    if cpu:
+        import oracle
+        b2 = oracle.Bvh2(v, t)
+        t0 = time.perf_counter()
+        _, st_cpu, _, _ = b2.walks(pts, 128, max_steps=256, eps=1e-3, step_scale=0.999, bmode=bc.mode,
+                                   bparams=bc.oracle_params(), seed=wp.seed, nthreads=threads)
+        dtc = time.perf_counter() - t0
+        out["cpu_oracle"] = {"seconds": dtc, "walk_steps_per_s": float(st_cpu.sum() / dtc), "threads": threads,
+                             "same_total_steps": bool(int(st_cpu.sum()) == steps)}
+    log(f"wos problem 1: {steps / dt / 1e9:.3f} G steps/s, rmse {out['rmse_vs_fp64']:.4g}")
+    return out
+
+
+def poisson_problem3(eng, torch, P, W) -> dict:
+    t0 = time.perf_counter()
+    v, t = P.make_mesh("mesh3")
+    bvh = eng.Bvh(v, t, leaf_size=4)
+    build_s = time.perf_counter() - t0
+    pts = P.interior_points(bvh, v, 262144, seed=0)
+    bc = P.poisson_boundary()
+    wp = W.WalkParams(max_steps=256, epsilon=1e-4, step_scale=0.999, seed=31, source=W.SOURCE_POISSON_SINES)
+    solver = W.Solver(bvh, bc, wp)
+    tp = torch.from_numpy(pts).cuda()
+    solver.solve(tp[:4096].contiguous(), 16, seed=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = solver.solve(tp, 512, seed=wp.seed)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    steps = int(st.steps.sum().item())
+    truth = P.poisson_solution_f64(pts)
+    mean = st.mean.cpu().numpy()
+    var = (st.m2 / (st.n - 1)).cpu().numpy()
+    err = mean - truth
+    z = err / np.sqrt(np.maximum(var / 512, 1e-300))
+    out = {"triangles": int(len(t)), "points": 262144, "walks_per_point": 512, "seconds": dt,
+           "walk_steps_per_s": steps / dt, "mean_steps_per_walk": steps / (262144 * 512),
+           "rmse_vs_fp64": float(np.sqrt(np.mean(err ** 2))), "mean_error": float(err.mean()),
+           "frac_within_3_sigma": float(np.mean(np.abs(z[var > 0]) <= 3.0)), "host_mesh_and_bvh_s": build_s,
+           "forced_walks": int(st.forced.sum().item())}
+    log(f"poisson problem 3: {steps / dt / 1e9:.3f} G steps/s, rmse {out['rmse_vs_fp64']:.4g}")
+    return out
+
+
+def cache_rates(eng, torch) -> dict:
+    """Training step (batch 2^18) and inference (2^20 points) of the benchmark
+    cache; MFMA work per point: 28,800 FLOP forward, 86,400 train."""
+    from n2wos_amd import cache as K
+    c = K.Cache(K.CacheConfig.for_bounds(np.full(3, -1.0), np.full(3, 1.0), seed=1))
+    g = torch.Generator(device="cuda").manual_seed(0)
+    x = (torch.rand((1 << 20, 3), device="cuda", generator=g) * 2 - 1).contiguous()
+    y = torch.sin(3 * x[:, 0]) * x[:, 1]
+    xb, yb = x[: 1 << 18].contiguous(), y[: 1 << 18].contiguous()
+    c.train_step(xb, yb)
+    train_ms = timed_events(torch, lambda: c.train_step(xb, yb), 10)
+    c.infer(x)
+    infer_ms = timed_events(torch, lambda: c.infer(x), 10)
+    tflops_train = 86400.0 * (1 << 18) / (train_ms * 1e-3) / 1e12
+    tflops_inf = 28800.0 * (1 << 20) / (infer_ms * 1e-3) / 1e12
+    return {"train_step_ms_batch_2^18": train_ms, "infer_ms_2^20": infer_ms,
+            "train_points_per_s": (1 << 18) / (train_ms * 1e-3), "infer_points_per_s": (1 << 20) / (infer_ms * 1e-3),
+            "train_mfma_TFLOPs": tflops_train, "infer_mfma_TFLOPs": tflops_inf,
+            "train_mfma_frac_fp16_dense": tflops_train / 2500.0, "infer_mfma_frac_fp16_dense": tflops_inf / 2500.0}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    import numpy as np
     import torch
 
     import __graft_entry__ as entry
@@ -190,10 +343,29 @@ def main():
         },
     }
 
+    threads = cpu_threads()
+    extra = line["extra"]
+    extra["closest_point_stress"] = closest_point_stress(eng, torch, P)
+    extra["wos_problem1"] = wos_problem1(eng, torch, P, W, not args.no_cpu, threads)
+    extra["cache"] = cache_rates(eng, torch)
+    if not args.quick:
+        extra["poisson_problem3"] = poisson_problem3(eng, torch, P, W)
+        from n2wos_amd import ttrmse
+        ttr_params = W.WalkParams(max_steps=256, epsilon=1e-4, step_scale=0.999, seed=2024)
+        r = ttrmse.run(bvh, verts, pts, bc, truth, ttr_params, plain_wpp=(320, 384, 448, 512), nc_wpp=(64, 256),
+                       lmc=((256, 32), (512, 64)), log=log)
+        extra["time_to_rmse"] = r
+        from n2wos_amd import serve as S
+        extra["serving"] = S.serving_workload(eng, n_requests=args.serve_requests, rate_per_s=args.serve_rate,
+                                              config=S.ServerConfig(hbm_budget_bytes=1 << 30, tick_walks=1 << 22),
+                                              log=log)
+        sv = extra["serving"]
+        log(f"serving: p50 {sv['latency_p50_s']:.2f} s p99 {sv['latency_p99_s']:.2f} s, "
+            f"{sv['walk_steps_per_s'] / 1e9:.3f} G steps/s")
+
     if not args.no_cpu:
         import oracle
         oracle.build()
-        threads = cpu_threads()
         b2 = oracle.Bvh2(verts, tris)
         # bounded sample: every k-th grid point, sized for about cpu_seconds
         pstride = max(1, n // 64)

@@ -301,12 +301,17 @@ int n2wos_cache_gradients(n2wos_cache* cache, const float* d_points, const float
  * missing (x 1.1, in units of 64, at most `max_round_walks`), until the
  * estimate is <= the target or `max_walks_per_point` is reached.
  *
- * Continuous batching: every n2wos_server_step() admits queued requests,
- * gives each mesh ONE persistent walker launch holding pieces of the current
- * rounds of all its active requests (max-min fair share of `tick_walks`
- * walks; whole points only), runs the meshes' launches concurrently on their
- * own streams with the device's blocks split by work, merges each piece into
- * its request's per-point FP64 (n, mean, M2), and retires finished requests.
+ * Continuous batching: each mesh has two batch slots, each with its own
+ * stream.  A slot's launch holds pieces of the current rounds of all the
+ * mesh's active requests (max-min fair share of `tick_walks` walks; whole
+ * points only): ONE persistent walker launch over a full-device grid, after
+ * which each piece is merged into its request's per-point FP64
+ * (n, mean, M2).  n2wos_server_step() admits queued requests, fills every
+ * free slot, waits for the oldest launch, retires every completed one, ends
+ * the rounds whose pieces are all merged (retiring finished requests) and
+ * refills the freed slots before it returns, so launches of all meshes
+ * overlap on the device and a new launch takes over the blocks that the
+ * tail of the previous one frees.
  * Walk j of request point i uses the Philox counter (step, j, rng_base + i)
  * under the mesh's seed, and a point's 64-walk units are merged in walk
  * order, so a request's results are bit-identical whatever else is in the
@@ -318,7 +323,7 @@ int n2wos_cache_gradients(n2wos_cache* cache, const float* d_points, const float
 typedef struct n2wos_server n2wos_server;
 typedef struct {
   int64_t hbm_budget_bytes;   /* device bytes for request state (one arena) */
-  int64_t tick_walks;         /* walks per mesh launch per step (>= 64) */
+  int64_t tick_walks;         /* walks per batch slot launch (>= 64) */
   int first_round_walks;      /* multiple of 64 */
   int max_round_walks;        /* multiple of 64 */
   int max_walks_per_point;    /* multiple of 64 */

@@ -5,22 +5,25 @@
 // Device state.  One arena (a single hipMalloc of the HBM budget) holds every
 // admitted request's points and per-point (n, mean, M2, steps, forced); the
 // host keeps a first-fit free list over it, so admission and retirement never
-// call the HIP allocator.  Each mesh owns a batch: capacity tick_walks / 64
-// entries of (point, Philox counter words, one 64-walk unit's statistics) plus
-// the walker workspace, allocated once when the mesh is added.
+// call the HIP allocator.  Each mesh owns two batch slots, each with its own
+// stream, event and buffers: capacity tick_walks / 64 entries of (point,
+// Philox counter words, one 64-walk unit's statistics) plus the walker
+// workspace, allocated once when the mesh is added.
 //
-// A step.  (1) Admit queued requests in arrival order while the arena has room.
-// (2) For each mesh, share its batch capacity max-min fairly among the
-// current rounds of its active requests (whole points; a point of a round
-// with k units takes k entries).  (3) Per mesh, on its own stream: upload the
-// piece table, gather the batch (entry -> request point, counter point id
-// rng_base + i and walk base round_base + 64 c), one persistent walker
-// launch (64 walks per entry), and merge each point's k entries, in walk
-// order, into the request's running statistics.  The device's blocks are
-// split between the meshes' launches in proportion to their walks, so the
-// launches run side by side.  (4) For requests whose round ended, a
-// fixed-order block reduction gives the estimated RMSE; finished requests are
-// copied out and their arena ranges freed.
+// A slot launch: share the slot's capacity max-min fairly among the current
+// rounds of the mesh's active requests (whole points; a point of a round with
+// k units takes k entries), upload the piece table, gather the batch (entry ->
+// request point, counter point id rng_base + i and walk base round_base +
+// 64 c), one persistent walker launch over a full-device grid (64 walks per
+// entry), and merge each point's k entries, in walk order, into the
+// request's running statistics.  Slots of all meshes run concurrently; the
+// blocks a draining launch frees go to the next one.
+//
+// A step: admit queued requests in arrival order while the arena has room,
+// fill free slots, wait for the oldest launch, retire every completed one;
+// for requests whose round is fully merged a fixed-order block reduction
+// gives the estimated RMSE; finished requests are copied out and their arena
+// ranges freed; freed slots are refilled at once.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -198,13 +201,14 @@ int64_t request_bytes(int n) {
   return align256(12 * p) + 4 * align256(8 * p) + align256(4 * p);
 }
 
-struct Mesh {
-  const n2wos_bvh* bvh = nullptr;
-  Boundary bc{};
-  n2wos_walk_params wp{};
+// One in-flight batch of a mesh: its own stream, buffers and completion event.
+struct Slot {
   hipStream_t stream = nullptr;
-  int cap = 0;            // batch entries
-  int full_grid = 1;
+  hipEvent_t done = nullptr;
+  bool busy = false;
+  int64_t seq = 0;        // launch order (the oldest busy slot is waited on first)
+  int entries = 0;
+  std::vector<int64_t> reqs;  // requests with a piece in this batch
   void* buf = nullptr;    // one allocation holding everything below
   float* bpts = nullptr;
   uint32_t* brng = nullptr;
@@ -214,9 +218,18 @@ struct Mesh {
   uint64_t* bsteps = nullptr;
   uint32_t* bforced = nullptr;
   void* ws = nullptr;
-  int64_t ws_bytes = 0;
   ServePiece* d_pieces = nullptr;
   ServePiece* h_pieces = nullptr;  // pinned staging
+};
+
+constexpr int kSlotsPerMesh = 2;
+
+struct Mesh {
+  const n2wos_bvh* bvh = nullptr;
+  Boundary bc{};
+  n2wos_walk_params wp{};
+  int cap = 0;            // batch entries per slot
+  Slot slot[kSlotsPerMesh];
   int64_t bytes = 0;
 };
 
@@ -239,6 +252,7 @@ struct Request {
   int wpp = 0;            // walks merged per point
   int round_walks = 0;    // walks per point of the current round
   int cursor = 0;         // points of the current round already batched
+  int inflight = 0;       // busy slots holding a piece of this request
   double est = -1.0;
   uint64_t total_steps = 0;
   double t_submit = -1.0, t_admit = -1.0, t_done = -1.0;
@@ -260,10 +274,11 @@ struct n2wos_server {
   std::deque<int64_t> queue;
   std::vector<int64_t> active;  // admission order
   int64_t next_id = 1;
-  int64_t ticks = 0, submitted = 0, done = 0, walks = 0;
+  int64_t ticks = 0, submitted = 0, done = 0, walks = 0, launches = 0;
   uint64_t steps = 0;
   int64_t fixed_bytes = 0;
   double busy = 0.0;
+  hipStream_t control = nullptr;  // admission uploads, estimates, result downloads
   n2::EstItem* d_est_items = nullptr;
   n2::EstItem* h_est_items = nullptr;
   double* d_est = nullptr;
@@ -284,7 +299,6 @@ void server_admit(n2wos_server* s) {
     if (static_cast<int>(s->active.size()) >= kMaxPieces) break;
     const int64_t off = s->arena->alloc(r.bytes);
     if (off < 0) break;  // first come, first served: later requests wait too
-    Mesh& m = *s->meshes[r.mesh];
     char* base = s->arena_base + off;
     const int64_t p = r.n;
     r.off = off;
@@ -299,12 +313,12 @@ void server_admit(n2wos_server* s) {
     r.d_steps = reinterpret_cast<uint64_t*>(base);
     base += align256(8 * p);
     r.d_forced = reinterpret_cast<uint32_t*>(base);
-    N2_HIP(hipMemcpyAsync(r.d_points, r.host_points.data(), 12 * p, hipMemcpyHostToDevice, m.stream));
+    N2_HIP(hipMemcpyAsync(r.d_points, r.host_points.data(), 12 * p, hipMemcpyHostToDevice, s->control));
     const int64_t stat_words = (r.bytes - align256(12 * p)) / 4;
-    hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((stat_words + 255) / 256)), dim3(256), 0, m.stream,
+    hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((stat_words + 255) / 256)), dim3(256), 0, s->control,
                        reinterpret_cast<uint32_t*>(r.d_n), stat_words);
     N2_HIP(hipGetLastError());
-    N2_HIP(hipStreamSynchronize(m.stream));  // the host copy of the points can go
+    N2_HIP(hipStreamSynchronize(s->control));  // the host copy of the points can go
     std::vector<float>().swap(r.host_points);
     r.state = N2WOS_REQ_ACTIVE;
     r.round_walks = s->cfg.first_round_walks;
@@ -337,157 +351,205 @@ std::vector<int> fair_share(const std::vector<int>& pts_left, const std::vector<
   return grant;
 }
 
+// Plans and launches one batch of mesh `mi` into its free slot `sl`; false if
+// no active request of the mesh has unscheduled points in its current round.
+bool launch_slot(n2wos_server* s, int mi, Slot& sl) {
+  Mesh& m = *s->meshes[mi];
+  std::vector<int64_t> ids;
+  for (int64_t id : s->active) {
+    const Request& r = *s->requests.at(id);
+    if (r.mesh == mi && r.cursor < r.n) ids.push_back(id);
+  }
+  if (ids.empty()) return false;
+  std::vector<int> left(ids.size()), units(ids.size());
+  for (size_t q = 0; q < ids.size(); ++q) {
+    const Request& r = *s->requests.at(ids[q]);
+    left[q] = r.n - r.cursor;
+    units[q] = r.round_walks / kUnit;
+  }
+  const std::vector<int> grant = fair_share(left, units, m.cap);
+  int entries = 0, points = 0, np = 0;
+  sl.reqs.clear();
+  for (size_t q = 0; q < ids.size(); ++q) {
+    if (grant[q] == 0) continue;
+    Request& r = *s->requests.at(ids[q]);
+    ServePiece& pc = sl.h_pieces[np++];
+    pc.points = r.d_points;
+    pc.n = r.d_n;
+    pc.mean = r.d_mean;
+    pc.m2 = r.d_m2;
+    pc.steps = r.d_steps;
+    pc.forced = r.d_forced;
+    pc.first = r.cursor;
+    pc.count = grant[q];
+    pc.units = units[q];
+    pc.batch_start = entries;
+    pc.point_start = points;
+    pc.id_base = r.rng_base;
+    pc.walk_base = static_cast<uint32_t>(r.wpp);
+    pc.pad = 0;
+    entries += grant[q] * units[q];
+    points += grant[q];
+    r.cursor += grant[q];
+    r.inflight += 1;
+    sl.reqs.push_back(r.id);
+  }
+  if (entries == 0) return false;
+  hipStream_t st = sl.stream;
+  N2_HIP(hipMemcpyAsync(sl.d_pieces, sl.h_pieces, sizeof(ServePiece) * np, hipMemcpyHostToDevice, st));
+  const int b = 256;
+  hipLaunchKernelGGL(k_serve_gather, dim3((entries + b - 1) / b), dim3(b), 0, st, sl.d_pieces, np, entries, sl.bpts,
+                     sl.brng);
+  N2_HIP(hipGetLastError());
+  WosRun w{};
+  w.d_points = sl.bpts;
+  w.n_points = entries;
+  w.walks_per_point = kUnit;
+  w.point_offset = 0;
+  w.d_point_rng = sl.brng;
+  w.max_steps = m.wp.max_steps;
+  w.epsilon = m.wp.epsilon;
+  w.step_scale = m.wp.step_scale;
+  w.seed = m.wp.seed;
+  w.rng_stream = m.wp.rng_stream;
+  w.source = m.wp.source;
+  w.max_blocks = 0;  // a full-device grid: a later launch takes over the blocks this one frees
+  w.d_n = sl.bn;
+  w.d_mean = sl.bmean;
+  w.d_m2 = sl.bm2;
+  w.d_steps = sl.bsteps;
+  w.d_forced = sl.bforced;
+  w.accumulate = 0;
+  w.d_workspace = sl.ws;
+  w.mode = kModePlain;
+  run_wos(m.bvh, m.bc, w, st);
+  hipLaunchKernelGGL(k_serve_merge, dim3((points + b - 1) / b), dim3(b), 0, st, sl.d_pieces, np, points, sl.bn,
+                     sl.bmean, sl.bm2, sl.bsteps, sl.bforced);
+  N2_HIP(hipGetLastError());
+  N2_HIP(hipEventRecord(sl.done, st));
+  sl.busy = true;
+  sl.seq = ++s->launches;
+  sl.entries = entries;
+  s->walks += static_cast<int64_t>(entries) * kUnit;
+  return true;
+}
+
+void launch_free_slots(n2wos_server* s) {
+  for (int mi = 0; mi < static_cast<int>(s->meshes.size()); ++mi)
+    for (Slot& sl : s->meshes[mi]->slot)
+      if (!sl.busy && !launch_slot(s, mi, sl)) break;
+}
+
+// Retires a completed slot: requests whose round is fully merged go to `round_done`.
+void retire(n2wos_server* s, Slot& sl, std::vector<int64_t>& round_done) {
+  for (int64_t id : sl.reqs) {
+    Request& r = *s->requests.at(id);
+    r.inflight -= 1;
+    if (r.inflight == 0 && r.cursor == r.n) round_done.push_back(id);
+  }
+  sl.reqs.clear();
+  sl.busy = false;
+}
+
+int finish_rounds(n2wos_server* s, const std::vector<int64_t>& round_done) {
+  if (round_done.empty()) return 0;
+  hipStream_t st = s->control;
+  const int k = static_cast<int>(round_done.size());
+  for (int q = 0; q < k; ++q) {
+    const Request& r = *s->requests.at(round_done[q]);
+    s->h_est_items[q] = EstItem{r.d_n, r.d_m2, r.n, 0};
+  }
+  N2_HIP(hipMemcpyAsync(s->d_est_items, s->h_est_items, sizeof(EstItem) * k, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_serve_estimate, dim3(k), dim3(kEstBlock), 0, st, s->d_est_items, s->d_est);
+  N2_HIP(hipGetLastError());
+  N2_HIP(hipMemcpyAsync(s->h_est, s->d_est, sizeof(double) * k, hipMemcpyDeviceToHost, st));
+  N2_HIP(hipStreamSynchronize(st));
+  int finished = 0;
+  for (int q = 0; q < k; ++q) {
+    Request& r = *s->requests.at(round_done[q]);
+    r.rounds += 1;
+    r.wpp += r.round_walks;
+    r.est = std::sqrt(std::max(0.0, s->h_est[q]));
+    const int room = s->cfg.max_walks_per_point - r.wpp;
+    if (r.est <= r.target || room <= 0) {
+      const size_t p = static_cast<size_t>(r.n);
+      r.r_n.resize(p);
+      r.r_mean.resize(p);
+      r.r_m2.resize(p);
+      r.r_steps.resize(p);
+      r.r_forced.resize(p);
+      N2_HIP(hipMemcpyAsync(r.r_n.data(), r.d_n, 8 * p, hipMemcpyDeviceToHost, st));
+      N2_HIP(hipMemcpyAsync(r.r_mean.data(), r.d_mean, 8 * p, hipMemcpyDeviceToHost, st));
+      N2_HIP(hipMemcpyAsync(r.r_m2.data(), r.d_m2, 8 * p, hipMemcpyDeviceToHost, st));
+      N2_HIP(hipMemcpyAsync(r.r_steps.data(), r.d_steps, 8 * p, hipMemcpyDeviceToHost, st));
+      N2_HIP(hipMemcpyAsync(r.r_forced.data(), r.d_forced, 4 * p, hipMemcpyDeviceToHost, st));
+      N2_HIP(hipStreamSynchronize(st));
+      uint64_t tot = 0;
+      for (uint64_t v : r.r_steps) tot += v;
+      r.total_steps = tot;
+      s->steps += tot;
+      s->arena->release(r.off, r.bytes);
+      r.off = -1;
+      r.d_points = nullptr;
+      r.state = N2WOS_REQ_DONE;
+      r.t_done = s->now();
+      s->active.erase(std::find(s->active.begin(), s->active.end(), r.id));
+      ++s->done;
+      ++finished;
+    } else {
+      // walks still missing by the estimate (variance ~ 1 / n), 10 % margin
+      const double ratio = (r.est / r.target) * (r.est / r.target);
+      const double want = std::ceil(static_cast<double>(r.wpp) * (ratio - 1.0) * 1.1);
+      int64_t w = (static_cast<int64_t>(want) + kUnit - 1) / kUnit * kUnit;
+      w = std::max<int64_t>(kUnit, std::min<int64_t>({w, s->cfg.max_round_walks, room}));
+      r.round_walks = static_cast<int>(w);
+      r.cursor = 0;
+    }
+  }
+  return finished;
+}
+
+// One step: admit, keep every free slot busy, wait for the oldest launch to
+// complete, retire every completed launch, finish the rounds that ended, and
+// refill the freed slots before returning.
 void server_step(n2wos_server* s, int* n_done) {
   const double t_begin = s->now();
   server_admit(s);
-  const int n_mesh = static_cast<int>(s->meshes.size());
-  std::vector<std::vector<int64_t>> per_mesh(n_mesh);
-  for (int64_t id : s->active) per_mesh[s->requests.at(id)->mesh].push_back(id);
-
-  // plan the pieces of every mesh
-  std::vector<int> n_pieces(n_mesh, 0), n_entries(n_mesh, 0), n_points(n_mesh, 0);
-  std::vector<int64_t> round_done;
-  int64_t total_entries = 0;
-  for (int mi = 0; mi < n_mesh; ++mi) {
-    Mesh& m = *s->meshes[mi];
-    const auto& ids = per_mesh[mi];
-    std::vector<int> left(ids.size()), units(ids.size());
-    for (size_t q = 0; q < ids.size(); ++q) {
-      const Request& r = *s->requests.at(ids[q]);
-      left[q] = r.n - r.cursor;
-      units[q] = r.round_walks / kUnit;
-    }
-    const std::vector<int> grant = fair_share(left, units, m.cap);
-    int entries = 0, points = 0, np = 0;
-    for (size_t q = 0; q < ids.size(); ++q) {
-      if (grant[q] == 0) continue;
-      Request& r = *s->requests.at(ids[q]);
-      ServePiece& pc = m.h_pieces[np++];
-      pc.points = r.d_points;
-      pc.n = r.d_n;
-      pc.mean = r.d_mean;
-      pc.m2 = r.d_m2;
-      pc.steps = r.d_steps;
-      pc.forced = r.d_forced;
-      pc.first = r.cursor;
-      pc.count = grant[q];
-      pc.units = units[q];
-      pc.batch_start = entries;
-      pc.point_start = points;
-      pc.id_base = r.rng_base;
-      pc.walk_base = static_cast<uint32_t>(r.wpp);
-      pc.pad = 0;
-      entries += grant[q] * units[q];
-      points += grant[q];
-      r.cursor += grant[q];
-      if (r.cursor == r.n) round_done.push_back(r.id);
-    }
-    n_pieces[mi] = np;
-    n_entries[mi] = entries;
-    n_points[mi] = points;
-    total_entries += entries;
-  }
-
-  // launch: one walker pool per mesh, blocks split by work
-  for (int mi = 0; mi < n_mesh; ++mi) {
-    if (n_entries[mi] == 0) continue;
-    Mesh& m = *s->meshes[mi];
-    hipStream_t st = m.stream;
-    N2_HIP(hipMemcpyAsync(m.d_pieces, m.h_pieces, sizeof(ServePiece) * n_pieces[mi], hipMemcpyHostToDevice, st));
-    const int b = 256;
-    hipLaunchKernelGGL(k_serve_gather, dim3((n_entries[mi] + b - 1) / b), dim3(b), 0, st, m.d_pieces, n_pieces[mi],
-                       n_entries[mi], m.bpts, m.brng);
-    N2_HIP(hipGetLastError());
-    WosRun w{};
-    w.d_points = m.bpts;
-    w.n_points = n_entries[mi];
-    w.walks_per_point = kUnit;
-    w.point_offset = 0;
-    w.d_point_rng = m.brng;
-    w.max_steps = m.wp.max_steps;
-    w.epsilon = m.wp.epsilon;
-    w.step_scale = m.wp.step_scale;
-    w.seed = m.wp.seed;
-    w.rng_stream = m.wp.rng_stream;
-    w.source = m.wp.source;
-    const double frac = static_cast<double>(n_entries[mi]) / static_cast<double>(total_entries);
-    w.max_blocks = std::max(1, static_cast<int>(std::floor(frac * m.full_grid)));
-    w.d_n = m.bn;
-    w.d_mean = m.bmean;
-    w.d_m2 = m.bm2;
-    w.d_steps = m.bsteps;
-    w.d_forced = m.bforced;
-    w.accumulate = 0;
-    w.d_workspace = m.ws;
-    w.mode = kModePlain;
-    run_wos(m.bvh, m.bc, w, st);
-    hipLaunchKernelGGL(k_serve_merge, dim3((n_points[mi] + b - 1) / b), dim3(b), 0, st, m.d_pieces, n_pieces[mi],
-                       n_points[mi], m.bn, m.bmean, m.bm2, m.bsteps, m.bforced);
-    N2_HIP(hipGetLastError());
-    s->walks += static_cast<int64_t>(n_entries[mi]) * kUnit;
-  }
-  for (int mi = 0; mi < n_mesh; ++mi)
-    if (n_entries[mi] > 0) N2_HIP(hipStreamSynchronize(s->meshes[mi]->stream));
-
-  // rounds that ended: estimate, then retire or plan the next round
+  launch_free_slots(s);
+  Slot* oldest = nullptr;
+  for (auto& m : s->meshes)
+    for (Slot& sl : m->slot)
+      if (sl.busy && (!oldest || sl.seq < oldest->seq)) oldest = &sl;
   int finished = 0;
-  if (!round_done.empty()) {
-    hipStream_t st = s->meshes[0]->stream;
-    const int k = static_cast<int>(round_done.size());
-    for (int q = 0; q < k; ++q) {
-      const Request& r = *s->requests.at(round_done[q]);
-      s->h_est_items[q] = EstItem{r.d_n, r.d_m2, r.n, 0};
-    }
-    N2_HIP(hipMemcpyAsync(s->d_est_items, s->h_est_items, sizeof(EstItem) * k, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_serve_estimate, dim3(k), dim3(kEstBlock), 0, st, s->d_est_items, s->d_est);
-    N2_HIP(hipGetLastError());
-    N2_HIP(hipMemcpyAsync(s->h_est, s->d_est, sizeof(double) * k, hipMemcpyDeviceToHost, st));
-    N2_HIP(hipStreamSynchronize(st));
-    for (int q = 0; q < k; ++q) {
-      Request& r = *s->requests.at(round_done[q]);
-      r.rounds += 1;
-      r.wpp += r.round_walks;
-      r.est = std::sqrt(std::max(0.0, s->h_est[q]));
-      const int room = s->cfg.max_walks_per_point - r.wpp;
-      if (r.est <= r.target || room <= 0) {
-        const size_t p = static_cast<size_t>(r.n);
-        r.r_n.resize(p);
-        r.r_mean.resize(p);
-        r.r_m2.resize(p);
-        r.r_steps.resize(p);
-        r.r_forced.resize(p);
-        N2_HIP(hipMemcpyAsync(r.r_n.data(), r.d_n, 8 * p, hipMemcpyDeviceToHost, st));
-        N2_HIP(hipMemcpyAsync(r.r_mean.data(), r.d_mean, 8 * p, hipMemcpyDeviceToHost, st));
-        N2_HIP(hipMemcpyAsync(r.r_m2.data(), r.d_m2, 8 * p, hipMemcpyDeviceToHost, st));
-        N2_HIP(hipMemcpyAsync(r.r_steps.data(), r.d_steps, 8 * p, hipMemcpyDeviceToHost, st));
-        N2_HIP(hipMemcpyAsync(r.r_forced.data(), r.d_forced, 4 * p, hipMemcpyDeviceToHost, st));
-        N2_HIP(hipStreamSynchronize(st));
-        uint64_t tot = 0;
-        for (uint64_t v : r.r_steps) tot += v;
-        r.total_steps = tot;
-        s->steps += tot;
-        s->arena->release(r.off, r.bytes);
-        r.off = -1;
-        r.d_points = nullptr;
-        r.state = N2WOS_REQ_DONE;
-        r.t_done = s->now();
-        s->active.erase(std::find(s->active.begin(), s->active.end(), r.id));
-        ++s->done;
-        ++finished;
-      } else {
-        // walks still missing by the estimate (variance ~ 1 / n), 10 % margin
-        const double ratio = (r.est / r.target) * (r.est / r.target);
-        const double want = std::ceil(static_cast<double>(r.wpp) * (ratio - 1.0) * 1.1);
-        int64_t w = (static_cast<int64_t>(want) + kUnit - 1) / kUnit * kUnit;
-        w = std::max<int64_t>(kUnit, std::min<int64_t>({w, s->cfg.max_round_walks, room}));
-        r.round_walks = static_cast<int>(w);
-        r.cursor = 0;
+  if (oldest) {
+    N2_HIP(hipEventSynchronize(oldest->done));
+    std::vector<int64_t> round_done;
+    for (auto& m : s->meshes)
+      for (Slot& sl : m->slot) {
+        if (!sl.busy) continue;
+        const hipError_t q = hipEventQuery(sl.done);
+        if (q == hipSuccess) {
+          retire(s, sl, round_done);
+        } else if (q != hipErrorNotReady) {
+          N2_HIP(q);
+        }
       }
-    }
+    finished = finish_rounds(s, round_done);
+    server_admit(s);
+    launch_free_slots(s);
   }
   ++s->ticks;
   s->busy += s->now() - t_begin;
   if (n_done) *n_done = finished;
+}
+
+void free_slot(Slot& sl) {
+  if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+  (void)hipFree(sl.buf);
+  (void)hipFree(sl.d_pieces);
+  (void)hipHostFree(sl.h_pieces);
+  if (sl.done) (void)hipEventDestroy(sl.done);
+  if (sl.stream) (void)hipStreamDestroy(sl.stream);
 }
 
 }  // namespace
@@ -516,6 +578,7 @@ extern "C" int n2wos_server_create(const n2wos_server_config* cfg, n2wos_server*
     }
     s->arena_base = static_cast<char*>(base);
     s->arena = std::make_unique<Arena>(cfg->hbm_budget_bytes);
+    N2_HIP(hipStreamCreateWithFlags(&s->control, hipStreamNonBlocking));
     N2_HIP(hipMalloc(&s->d_est_items, sizeof(EstItem) * kMaxPieces));
     N2_HIP(hipMalloc(&s->d_est, sizeof(double) * kMaxPieces));
     N2_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_est_items), sizeof(EstItem) * kMaxPieces));
@@ -529,18 +592,15 @@ extern "C" int n2wos_server_create(const n2wos_server_config* cfg, n2wos_server*
 extern "C" int n2wos_server_destroy(n2wos_server* s) {
   return guarded([&]() -> int {
     if (!s) return N2WOS_OK;
-    for (auto& m : s->meshes) {
-      if (m->stream) (void)hipStreamSynchronize(m->stream);
-      (void)hipFree(m->buf);
-      (void)hipFree(m->d_pieces);
-      (void)hipHostFree(m->h_pieces);
-      if (m->stream) (void)hipStreamDestroy(m->stream);
-    }
+    for (auto& m : s->meshes)
+      for (Slot& sl : m->slot) free_slot(sl);
+    if (s->control) (void)hipStreamSynchronize(s->control);
     (void)hipFree(s->arena_base);
     (void)hipFree(s->d_est_items);
     (void)hipFree(s->d_est);
     (void)hipHostFree(s->h_est_items);
     (void)hipHostFree(s->h_est);
+    if (s->control) (void)hipStreamDestroy(s->control);
     delete s;
     return N2WOS_OK;
   });
@@ -561,37 +621,44 @@ extern "C" int n2wos_server_add_mesh(n2wos_server* s, const n2wos_bvh* bvh, cons
     m->wp.event_begin = nullptr;
     m->wp.event_end = nullptr;
     m->cap = static_cast<int>(s->cfg.tick_walks / kUnit);
-    m->full_grid = wos_full_grid(params->source);
     const int64_t c = m->cap;
-    m->ws_bytes = wos_workspace_bytes(m->cap, kUnit);
+    const int64_t ws_bytes = wos_workspace_bytes(m->cap, kUnit);
     const int64_t sz[8] = {align256(12 * c), align256(8 * c), align256(8 * c), align256(8 * c),
-                           align256(8 * c), align256(8 * c), align256(4 * c), align256(m->ws_bytes)};
+                           align256(8 * c), align256(8 * c), align256(4 * c), align256(ws_bytes)};
     int64_t total = 0;
     for (int64_t v : sz) total += v;
-    if (hipMalloc(&m->buf, total) != hipSuccess) {
-      (void)hipGetLastError();
-      throw Error(N2WOS_ERR_NOMEM, "cannot allocate the mesh batch");
+    try {
+      for (Slot& sl : m->slot) {
+        if (hipMalloc(&sl.buf, total) != hipSuccess) {
+          (void)hipGetLastError();
+          throw Error(N2WOS_ERR_NOMEM, "cannot allocate the mesh batch");
+        }
+        char* p = static_cast<char*>(sl.buf);
+        sl.bpts = reinterpret_cast<float*>(p);
+        p += sz[0];
+        sl.brng = reinterpret_cast<uint32_t*>(p);
+        p += sz[1];
+        sl.bn = reinterpret_cast<double*>(p);
+        p += sz[2];
+        sl.bmean = reinterpret_cast<double*>(p);
+        p += sz[3];
+        sl.bm2 = reinterpret_cast<double*>(p);
+        p += sz[4];
+        sl.bsteps = reinterpret_cast<uint64_t*>(p);
+        p += sz[5];
+        sl.bforced = reinterpret_cast<uint32_t*>(p);
+        p += sz[6];
+        sl.ws = p;
+        N2_HIP(hipMalloc(&sl.d_pieces, sizeof(ServePiece) * kMaxPieces));
+        N2_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_pieces), sizeof(ServePiece) * kMaxPieces));
+        N2_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+        N2_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+      }
+    } catch (...) {
+      for (Slot& sl : m->slot) free_slot(sl);
+      throw;
     }
-    char* p = static_cast<char*>(m->buf);
-    m->bpts = reinterpret_cast<float*>(p);
-    p += sz[0];
-    m->brng = reinterpret_cast<uint32_t*>(p);
-    p += sz[1];
-    m->bn = reinterpret_cast<double*>(p);
-    p += sz[2];
-    m->bmean = reinterpret_cast<double*>(p);
-    p += sz[3];
-    m->bm2 = reinterpret_cast<double*>(p);
-    p += sz[4];
-    m->bsteps = reinterpret_cast<uint64_t*>(p);
-    p += sz[5];
-    m->bforced = reinterpret_cast<uint32_t*>(p);
-    p += sz[6];
-    m->ws = p;
-    N2_HIP(hipMalloc(&m->d_pieces, sizeof(ServePiece) * kMaxPieces));
-    N2_HIP(hipHostMalloc(reinterpret_cast<void**>(&m->h_pieces), sizeof(ServePiece) * kMaxPieces));
-    N2_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
-    m->bytes = total + static_cast<int64_t>(sizeof(ServePiece)) * kMaxPieces;
+    m->bytes = kSlotsPerMesh * (total + static_cast<int64_t>(sizeof(ServePiece)) * kMaxPieces);
     s->fixed_bytes += m->bytes;
     *mesh_id = static_cast<int>(s->meshes.size());
     s->meshes.push_back(std::move(m));

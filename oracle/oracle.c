@@ -1150,6 +1150,41 @@ static void orc_unit(const orc_grid* g, const float* p, double u[3]) {
   }
 }
 
+/* FP32 versions of the unit-cube mapping and the cell corners, with the
+ * product kernels' operation order (used by the FP16-rounding variant, whose
+ * encoding must round like the product's FP32 encoding). */
+static void orc_unit_f32(const orc_grid* g, const float* p, float u[3]) {
+  for (int a = 0; a < 3; ++a) {
+    const float v = (p[a] - (float)g->in_min[a]) * (float)g->in_scale[a];
+    u[a] = fminf(1.0f, fmaxf(0.0f, v));
+  }
+}
+
+static void orc_corners_f32(const orc_grid* g, int l, const float u[3], uint32_t idx[8], float w[8]) {
+  const int n = g->res[l];
+  int c[3];
+  float f[3];
+  for (int a = 0; a < 3; ++a) {
+    const float sa = u[a] * (float)n;
+    int ci = (int)floorf(sa);
+    ci = ci < 0 ? 0 : (ci > n - 1 ? n - 1 : ci);
+    c[a] = ci;
+    f[a] = sa - (float)ci;
+  }
+  const uint32_t stride = (uint32_t)(n + 1);
+  for (int k = 0; k < 8; ++k) {
+    const uint32_t x = (uint32_t)(c[0] + (k & 1)), y = (uint32_t)(c[1] + ((k >> 1) & 1)),
+                   z = (uint32_t)(c[2] + ((k >> 2) & 1));
+    const uint32_t local = g->dense[l] ? x + y * stride + z * stride * stride
+                                       : (x ^ (y * 2654435761u) ^ (z * 805459861u)) & (g->table_size - 1u);
+    idx[k] = g->offset[l] + local;
+    const float wx = (k & 1) ? f[0] : 1.0f - f[0];
+    const float wy = ((k >> 1) & 1) ? f[1] : 1.0f - f[1];
+    const float wz = ((k >> 2) & 1) ? f[2] : 1.0f - f[2];
+    w[k] = wx * wy * wz;
+  }
+}
+
 static void orc_grid_init(orc_grid* g, int n_levels, int log2_t, int base_res, double growth, const float* in_min,
                           const float* in_extent) {
   g->n_levels = n_levels;
@@ -1182,20 +1217,36 @@ double orc_round_half(double v) {
  * stay in double. */
 static double orc_mlp_fwd_ex(const orc_grid* g, const float* grid, const float* wt, const float* p, double x[32],
                              double h[4][64], int half) {
-  double u[3];
-  orc_unit(g, p, u);
   for (int k = 0; k < 32; ++k) x[k] = 0.0;
-  for (int l = 0; l < g->n_levels; ++l) {
-    uint32_t idx[8];
-    double w[8];
-    orc_corners(g, l, u, idx, w);
-    for (int c = 0; c < 8; ++c) {
-      x[2 * l] += w[c] * (double)grid[2 * (size_t)idx[c]];
-      x[2 * l + 1] += w[c] * (double)grid[2 * (size_t)idx[c] + 1];
+  if (half) {
+    /* FP32 encoding in the product's order, then binary16 */
+    float u[3];
+    orc_unit_f32(g, p, u);
+    for (int l = 0; l < g->n_levels; ++l) {
+      uint32_t idx[8];
+      float w[8];
+      orc_corners_f32(g, l, u, idx, w);
+      float f0 = 0.0f, f1 = 0.0f;
+      for (int c = 0; c < 8; ++c) {
+        f0 += w[c] * grid[2 * (size_t)idx[c]];
+        f1 += w[c] * grid[2 * (size_t)idx[c] + 1];
+      }
+      x[2 * l] = orc_round_half((double)f0);
+      x[2 * l + 1] = orc_round_half((double)f1);
+    }
+  } else {
+    double u[3];
+    orc_unit(g, p, u);
+    for (int l = 0; l < g->n_levels; ++l) {
+      uint32_t idx[8];
+      double w[8];
+      orc_corners(g, l, u, idx, w);
+      for (int c = 0; c < 8; ++c) {
+        x[2 * l] += w[c] * (double)grid[2 * (size_t)idx[c]];
+        x[2 * l + 1] += w[c] * (double)grid[2 * (size_t)idx[c] + 1];
+      }
     }
   }
-  if (half)
-    for (int k = 0; k < 32; ++k) x[k] = orc_round_half(x[k]);
   const double* in = x;
   int nin = 32;
   const int offs[4] = {0, 2048, 6144, 10240};
@@ -1284,11 +1335,19 @@ static double orc_cache_grads_ex(int n_levels, int log2_t, int base_res, double 
     }
     /* dh now holds dL/dx (times the scale) */
     double u[3];
+    float uf[3];
     orc_unit(&g, pts + 3 * s, u);
+    orc_unit_f32(&g, pts + 3 * s, uf);
     for (int l = 0; l < g.n_levels; ++l) {
       uint32_t idx[8];
       double w[8];
-      orc_corners(&g, l, u, idx, w);
+      if (half) {
+        float wf[8];
+        orc_corners_f32(&g, l, uf, idx, wf);
+        for (int c = 0; c < 8; ++c) w[c] = (double)wf[c];
+      } else {
+        orc_corners(&g, l, u, idx, w);
+      }
       for (int c = 0; c < 8; ++c) {
         grid_grad[2 * (size_t)idx[c]] += w[c] * dh[2 * l] / scale;
         grid_grad[2 * (size_t)idx[c] + 1] += w[c] * dh[2 * l + 1] / scale;
@@ -1305,8 +1364,9 @@ double orc_cache_grads(int n_levels, int log2_t, int base_res, double growth, co
                             grid_grad, w_grad, 0.0);
 }
 
-/* The same gradients with the product's binary16 rounding points (weights,
- * features, activations, scaled dL/dy and dZ); loss_scale as the product's. */
+/* The same gradients with the product's FP32 encoding and binary16 rounding
+ * points (weights, features, activations, scaled dL/dy and dZ); loss_scale as
+ * the product's. */
 double orc_cache_grads_fp16(int n_levels, int log2_t, int base_res, double growth, const float* in_min,
                             const float* in_extent, const float* grid, const float* wt, const float* pts,
                             const float* targets, long n, double* grid_grad, double* w_grad, double loss_scale) {

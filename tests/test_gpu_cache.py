@@ -1,9 +1,10 @@
 """Learned cache on the GPU against the FP64 oracle: layout; forward and MSE
-gradients against the oracle variant that rounds to binary16 where the
-kernels do (FP16 MFMA operands, FP32 accumulation: relative L2 error <= 5e-3
-per parameter group) and against the exact FP64 oracle (the FP16 operands
-alone move a gradient by a few percent when the loss terms cancel: <= 6e-2);
-Adam's first step in closed form; an overfit test on x^2 - y^2."""
+gradients against the oracle variant that encodes in FP32 and rounds to
+binary16 where the kernels do (FP16 MFMA operands, FP32 accumulation:
+relative L2 error <= 1e-3 per weight group, 2e-3 for the grid) and against
+the exact FP64 oracle (FP16 operands alone move these gradients by a few
+percent, because the loss terms of random targets cancel: <= 6e-2); Adam's
+first step in closed form; an overfit test on x^2 - y^2."""
 import numpy as np
 import pytest
 
@@ -71,10 +72,10 @@ def test_gradients_match_fp64_oracle(eng, orc, torch_gpu, small):
           "grid %.2e" % rel(gg, ogg), flush=True)
     assert abs(loss - oloss) < 2e-2 * oloss
     for lo, hi in groups:
-        assert rel(wg[lo:hi], hwg[lo:hi]) < 5e-3, (lo, hi)
+        assert rel(wg[lo:hi], hwg[lo:hi]) < 1e-3, (lo, hi)
         assert rel(wg[lo:hi], owg[lo:hi]) < 6e-2, (lo, hi)
     assert np.all(wg[14400:] == 0)
-    assert rel(gg, hgg) < 5e-3
+    assert rel(gg, hgg) < 2e-3
     assert rel(gg, ogg) < 6e-2
 
 
