@@ -108,8 +108,10 @@ int n2wos_closest_point_bruteforce(const float* verts, int nv, const uint32_t* t
 
 /* -------------------------------------------------------------------- BVH */
 /* Builds the BVH on the host and uploads it to the current device.
- * build_method: NULL, "median" or "custom" (median split, leaves of at most
- * leaf_size triangles, leaf_size in [1, 31]); others -> N2WOS_ERR_UNSUPPORTED. */
+ * build_method: NULL, "median" or "custom" (split at the centroid median of
+ * the largest extent), or "sah" / "surface_area_heuristic" (binned surface
+ * area heuristic); leaves of at most leaf_size triangles, leaf_size in
+ * [1, 31]; others -> N2WOS_ERR_UNSUPPORTED. */
 int n2wos_bvh_create(const float* verts, int nv, const uint32_t* tris, int nt, int leaf_size,
                      const char* build_method, n2wos_bvh** out);
 int n2wos_bvh_destroy(n2wos_bvh* bvh);
@@ -157,7 +159,8 @@ int n2wos_bvh_query_device_counted(const n2wos_bvh* bvh, const float* d_points, 
  * NULL to get *node_count; tris_out holds nt records of 48 bytes.
  * *prune_margin receives the absolute pruning margin. */
 int n2wos_bvh_build_host(const float* verts, int nv, const uint32_t* tris, int nt, int leaf_size,
-                         void* nodes_out, int64_t* node_count, void* tris_out, float* prune_margin);
+                         const char* build_method, void* nodes_out, int64_t* node_count, void* tris_out,
+                         float* prune_margin);
 /* Host copies of the device arrays: nodes (128 B each, node_count of them)
  * and triangle records (48 B each).  Either pointer may be NULL. */
 int n2wos_bvh_export(const n2wos_bvh* bvh, void* nodes_out, void* tris_out);

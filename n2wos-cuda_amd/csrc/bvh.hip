@@ -150,6 +150,19 @@ void launch_closest_point(const n2wos_bvh* bvh, const float* d_points, int n, fl
 
 using namespace n2;
 
+namespace {
+// Build-method names: the median split ("median"; the reference's own host
+// builder is "custom") and the binned SAH ("sah", the reference's cuBQL
+// naming, src/cuda/cubql_bvh.cu:114-122 for the aliases).
+std::string canonical_method(const char* build_method) {
+  std::string m = build_method ? build_method : "median";
+  if (m == "custom" || m == "custom_cuda_bvh") m = "median";
+  if (m == "surface_area_heuristic" || m == "cubql_cuda_sah") m = "sah";
+  N2_REQUIRE(m == "median" || m == "sah", N2WOS_ERR_UNSUPPORTED, "unsupported BVH build method: " + m);
+  return m;
+}
+}  // namespace
+
 extern "C" int n2wos_bvh_create(const float* verts, int nv, const uint32_t* tris, int nt, int leaf_size,
                                 const char* build_method, n2wos_bvh** out) {
   return guarded([&]() -> int {
@@ -157,11 +170,9 @@ extern "C" int n2wos_bvh_create(const float* verts, int nv, const uint32_t* tris
     *out = nullptr;
     N2_REQUIRE(verts && tris, N2WOS_ERR_NULL, "mesh arrays must not be NULL");
     N2_REQUIRE(nv > 0 && nt > 0, N2WOS_ERR_EMPTY, "mesh has no vertices or no triangles");
-    std::string method = build_method ? build_method : "median";
-    if (method == "custom" || method == "custom_cuda_bvh") method = "median";
-    N2_REQUIRE(method == "median", N2WOS_ERR_UNSUPPORTED, "unsupported BVH build method: " + method);
+    const std::string method = canonical_method(build_method);
     const auto t0 = std::chrono::steady_clock::now();
-    HostBvh4 host = build_bvh4(verts, nv, tris, nt, leaf_size);
+    HostBvh4 host = build_bvh4(verts, nv, tris, nt, leaf_size, method == "sah");
     const auto t1 = std::chrono::steady_clock::now();
     auto* h = new n2wos_bvh();
     try {
@@ -237,11 +248,12 @@ extern "C" int n2wos_bvh_get_stats(const n2wos_bvh* bvh, n2wos_bvh_stats* out) {
 }
 
 extern "C" int n2wos_bvh_build_host(const float* verts, int nv, const uint32_t* tris, int nt, int leaf_size,
-                                    void* nodes_out, int64_t* node_count, void* tris_out, float* prune_margin) {
+                                    const char* build_method, void* nodes_out, int64_t* node_count, void* tris_out,
+                                    float* prune_margin) {
   return guarded([&]() -> int {
     N2_REQUIRE(verts && tris && node_count, N2WOS_ERR_NULL, "NULL argument");
     N2_REQUIRE(nv > 0 && nt > 0, N2WOS_ERR_EMPTY, "mesh has no vertices or no triangles");
-    HostBvh4 host = build_bvh4(verts, nv, tris, nt, leaf_size);
+    HostBvh4 host = build_bvh4(verts, nv, tris, nt, leaf_size, canonical_method(build_method) == "sah");
     *node_count = static_cast<int64_t>(host.nodes.size());
     if (prune_margin) *prune_margin = host.delta;
     if (nodes_out) std::memcpy(nodes_out, host.nodes.data(), host.nodes.size() * sizeof(Bvh4Node));

@@ -52,6 +52,6 @@ struct HostBvh4 {
 // nth_element, leaves of at most leaf_size triangles -- the reference builder
 // src/cuda/bvh_build_host.cpp:74-120) and collapses it into 4-wide nodes by
 // repeatedly opening the child with the largest surface area.
-HostBvh4 build_bvh4(const float* verts, int nv, const uint32_t* tris, int nt, int leaf_size);
+HostBvh4 build_bvh4(const float* verts, int nv, const uint32_t* tris, int nt, int leaf_size, bool sah = false);
 
 }  // namespace n2

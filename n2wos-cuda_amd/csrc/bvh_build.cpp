@@ -1,4 +1,7 @@
-// bvh_build.cpp -- host construction of the 4-wide BVH (see bvh.hpp).
+// bvh_build.cpp -- host construction of the 4-wide BVH (see bvh.hpp): a
+// binary tree split at the centroid median of the largest extent (the
+// reference's build, src/cuda/bvh_build_host.cpp:74-120) or by a binned
+// surface-area heuristic, collapsed to 4-wide nodes.
 #include <algorithm>
 #include <cmath>
 #include <limits>
@@ -48,7 +51,63 @@ struct Binary {
   int depth = 0;
 };
 
-int build_binary(Binary& out, const std::vector<Prim>& prims, int begin, int end, int depth, int leaf_size) {
+// Binned surface-area-heuristic split (32 bins per axis over the centroid
+// box): minimises area(L) n(L) + area(R) n(R); partitions order[begin, end)
+// and returns the split index, or -1 when no bin boundary separates the
+// centroids (the caller then splits at the median).
+int sah_partition(std::vector<int>& order, const std::vector<Prim>& prims, int begin, int end, const Box& cbox) {
+  constexpr int kBins = 32;
+  float best_cost = std::numeric_limits<float>::infinity();
+  int best_axis = -1, best_bin = -1;
+  for (int a = 0; a < 3; ++a) {
+    const float lo = cbox.lo[a], ext = cbox.hi[a] - cbox.lo[a];
+    if (!(ext > 0.0f)) continue;
+    Box bb[kBins];
+    int bn[kBins] = {0};
+    const float k = kBins / ext;
+    for (int i = begin; i < end; ++i) {
+      const Prim& p = prims[static_cast<size_t>(order[static_cast<size_t>(i)])];
+      const int b = std::min(kBins - 1, static_cast<int>((p.cen[a] - lo) * k));
+      bn[b] += 1;
+      bb[b].grow(p.box);
+    }
+    float right_area[kBins];
+    int right_n[kBins];
+    Box acc;
+    int n = 0;
+    for (int b = kBins - 1; b > 0; --b) {
+      if (bn[b]) acc.grow(bb[b]);
+      n += bn[b];
+      right_area[b] = n ? acc.area() : 0.0f;
+      right_n[b] = n;
+    }
+    Box lacc;
+    int ln = 0;
+    for (int b = 0; b < kBins - 1; ++b) {
+      if (bn[b]) lacc.grow(bb[b]);
+      ln += bn[b];
+      const int rn = right_n[b + 1];
+      if (ln == 0 || rn == 0) continue;
+      const float cost = lacc.area() * static_cast<float>(ln) + right_area[b + 1] * static_cast<float>(rn);
+      if (cost < best_cost) {
+        best_cost = cost;
+        best_axis = a;
+        best_bin = b;
+      }
+    }
+  }
+  if (best_axis < 0) return -1;
+  const float lo = cbox.lo[best_axis], k = kBins / (cbox.hi[best_axis] - cbox.lo[best_axis]);
+  auto it = std::partition(order.begin() + begin, order.begin() + end, [&](int id) {
+    const Prim& p = prims[static_cast<size_t>(id)];
+    return std::min(kBins - 1, static_cast<int>((p.cen[best_axis] - lo) * k)) <= best_bin;
+  });
+  const int mid = static_cast<int>(it - order.begin());
+  return (mid > begin && mid < end) ? mid : -1;
+}
+
+int build_binary(Binary& out, const std::vector<Prim>& prims, int begin, int end, int depth, int leaf_size,
+                 bool sah) {
   out.depth = std::max(out.depth, depth);
   Box box, cbox;
   for (int i = begin; i < end; ++i) {
@@ -65,15 +124,20 @@ int build_binary(Binary& out, const std::vector<Prim>& prims, int begin, int end
     out.nodes[static_cast<size_t>(idx)].count = count;
     return idx;
   }
-  const float ex = cbox.hi[0] - cbox.lo[0], ey = cbox.hi[1] - cbox.lo[1], ez = cbox.hi[2] - cbox.lo[2];
-  const int axis = (ex >= ey && ex >= ez) ? 0 : ((ey >= ex && ey >= ez) ? 1 : 2);
-  const int mid = begin + count / 2;
-  auto first = out.order.begin() + begin;
-  std::nth_element(first, out.order.begin() + mid, out.order.begin() + end, [&](int a, int b) {
-    return prims[static_cast<size_t>(a)].cen[axis] < prims[static_cast<size_t>(b)].cen[axis];
-  });
-  const int l = build_binary(out, prims, begin, mid, depth + 1, leaf_size);
-  const int r = build_binary(out, prims, mid, end, depth + 1, leaf_size);
+  int mid = -1;
+  if (sah) mid = sah_partition(out.order, prims, begin, end, cbox);
+  if (mid < 0) {
+    // median split on the largest centroid extent
+    const float ex = cbox.hi[0] - cbox.lo[0], ey = cbox.hi[1] - cbox.lo[1], ez = cbox.hi[2] - cbox.lo[2];
+    const int axis = (ex >= ey && ex >= ez) ? 0 : ((ey >= ex && ey >= ez) ? 1 : 2);
+    mid = begin + count / 2;
+    auto first = out.order.begin() + begin;
+    std::nth_element(first, out.order.begin() + mid, out.order.begin() + end, [&](int a, int b) {
+      return prims[static_cast<size_t>(a)].cen[axis] < prims[static_cast<size_t>(b)].cen[axis];
+    });
+  }
+  const int l = build_binary(out, prims, begin, mid, depth + 1, leaf_size, sah);
+  const int r = build_binary(out, prims, mid, end, depth + 1, leaf_size, sah);
   out.nodes[static_cast<size_t>(idx)].left = l;
   out.nodes[static_cast<size_t>(idx)].right = r;
   return idx;
@@ -138,7 +202,7 @@ int collapse(HostBvh4& out, const Binary& bin, int bi, int depth) {
 
 }  // namespace
 
-HostBvh4 build_bvh4(const float* verts, int nv, const uint32_t* tris, int nt, int leaf_size) {
+HostBvh4 build_bvh4(const float* verts, int nv, const uint32_t* tris, int nt, int leaf_size, bool sah) {
   if (nv <= 0 || nt <= 0) throw std::invalid_argument("mesh has no vertices or no triangles");
   if (leaf_size < 1 || leaf_size > kMaxLeafSize) throw std::invalid_argument("leaf size must be in [1, 31]");
   if (nt >= (1 << 26)) throw std::invalid_argument("at most 2^26 - 1 triangles are supported");
@@ -165,7 +229,7 @@ HostBvh4 build_bvh4(const float* verts, int nv, const uint32_t* tris, int nt, in
   bin.order.resize(static_cast<size_t>(nt));
   std::iota(bin.order.begin(), bin.order.end(), 0);
   bin.nodes.reserve(static_cast<size_t>(2 * (nt / std::max(1, leaf_size)) + 3));
-  build_binary(bin, prims, 0, nt, 0, leaf_size);
+  build_binary(bin, prims, 0, nt, 0, leaf_size, sah);
 
   HostBvh4 out;
   out.leaf_size = leaf_size;

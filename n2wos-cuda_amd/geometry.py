@@ -73,16 +73,17 @@ def closest_point_bruteforce(verts, tris, points, nthreads: int = 0):
     return d2, cp, tri
 
 
-def build_host(verts, tris, leaf_size: int = 8):
+def build_host(verts, tris, leaf_size: int = 8, build_method: str = "median"):
     """Node and triangle-record arrays exactly as uploaded (no device needed)."""
     v = np.ascontiguousarray(verts, np.float32)
     t = np.ascontiguousarray(tris, np.uint32)
     n = C.c_int64()
     delta = C.c_float()
-    check(lib().n2wos_bvh_build_host(_ptr(v), len(v), _ptr(t), len(t), leaf_size, None, C.byref(n), None, None))
+    m = build_method.encode()
+    check(lib().n2wos_bvh_build_host(_ptr(v), len(v), _ptr(t), len(t), leaf_size, m, None, C.byref(n), None, None))
     nodes = np.zeros(n.value * NODE_BYTES, np.uint8)
     recs = np.zeros(len(t) * TRI_BYTES, np.uint8)
-    check(lib().n2wos_bvh_build_host(_ptr(v), len(v), _ptr(t), len(t), leaf_size, _ptr(nodes), C.byref(n),
+    check(lib().n2wos_bvh_build_host(_ptr(v), len(v), _ptr(t), len(t), leaf_size, m, _ptr(nodes), C.byref(n),
                                      _ptr(recs), C.byref(delta)))
     return nodes, recs, delta.value
 

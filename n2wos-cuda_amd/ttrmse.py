@@ -7,10 +7,15 @@ Cache training is online and counted in the NC / 2LMC wall time: coupled walks
 from the evaluation points (their own Philox stream, never reused for an
 estimate) give (X_m, W - S_m) pairs, unbiased labels of u(X_m); the cache
 takes Adam steps on random batches of them, then is frozen for the estimate.
+
+Several cache setups can be compared in one run (depth m, grid levels,
+labels per point, Adam steps); each reports its own training time, the cache
+error at held-out walk positions X_m, and the NC and NC + 2LMC runs.
 """
 from __future__ import annotations
 
 import time
+from dataclasses import dataclass, field
 
 import numpy as np
 
@@ -19,15 +24,36 @@ def _rmse(mean, truth):
     return float(np.sqrt(np.mean((mean - truth) ** 2)))
 
 
+@dataclass
+class CacheSetup:
+    name: str
+    depth: int = 4
+    cache_kw: dict = field(default_factory=dict)  # CacheConfig overrides (n_levels, ...)
+    train_pairs: int = 32                          # coupled label walks per point
+    train_steps: int = 200
+    batch: int = 1 << 18
+    nc_wpp: tuple = (64, 128, 256)
+    lmc: tuple = ((256, 16), (512, 16))            # (coarse samples, coupled pairs) per point
+
+
+# The learned-cache configuration of the benchmark problem (16 levels x 2
+# features, T = 2^19, N_min 16, growth 1.5; 4 x 64 ReLU MLP; depth 4; 32
+# label walks per point; 16 coupled pairs per point for 2LMC) and the best
+# setup found on the same problem (depth 1, 8 levels, 100 Adam steps).
+DEFAULT_SETUPS = (
+    CacheSetup("depth4_levels16", depth=4, train_steps=200, nc_wpp=(64, 256), lmc=((256, 16), (512, 16))),
+    CacheSetup("depth1_levels8", depth=1, cache_kw={"n_levels": 8}, train_steps=100, nc_wpp=(128, 192, 256),
+               lmc=()),
+)
+
+
 def run(bvh, verts, points, boundary, truth, params, tau=1e-2, plain_wpp=(256, 320, 384, 448, 512),
-        depth=4, train_pairs=4, train_steps=200, batch=1 << 18, nc_wpp=(32, 64, 128, 256),
-        lmc=((128, 16), (192, 16), (256, 16), (256, 32), (384, 32), (512, 64)), cache_config=None, seed=11,
-        log=print):
+        setups=DEFAULT_SETUPS, seed=11, log=print):
     import torch
 
     from . import cache as K, nc as N, wos as W
     tp = torch.from_numpy(points).cuda()
-    out = {"tau": tau, "points": int(len(points)), "depth": depth}
+    out = {"tau": tau, "points": int(len(points))}
 
     def timed(fn):
         torch.cuda.synchronize()
@@ -47,66 +73,75 @@ def run(bvh, verts, points, boundary, truth, params, tau=1e-2, plain_wpp=(256, 3
         log(f"plain wpp={wpp} t={dt:.3f}s rmse={r:.4g}")
     out["plain"] = plain
     ok = [p["seconds"] for p in plain if p["rmse"] <= tau]
-    out["plain_time_to_rmse"] = min(ok) if ok else None
+    pt = min(ok) if ok else None
+    out["plain_time_to_rmse"] = pt
 
-    # ---------------------------------------------------------------- train
     ns = N.NcSolver(bvh, boundary, params)
-    cfg = cache_config or K.CacheConfig.for_bounds(verts.min(0), verts.max(0), seed=seed)
-    cache = K.Cache(cfg)
+    best = {}
+    out["setups"] = []
+    for su in setups:
+        res = {"name": su.name, "depth": su.depth, "cache": dict(su.cache_kw), "train_pairs": su.train_pairs,
+               "train_steps": su.train_steps, "batch": su.batch}
+        cfg = K.CacheConfig.for_bounds(verts.min(0), verts.max(0), seed=seed, **su.cache_kw)
+        cache = K.Cache(cfg)
 
-    def train():
-        b = ns.walks(tp, train_pairs, N.NC_COUPLED, depth, stream_id=7, seed=seed)
-        x, y = ns.training_pairs(b)
-        g = torch.Generator(device="cuda").manual_seed(seed)
-        n = x.shape[0]
-        for _ in range(train_steps):
-            idx = torch.randint(0, n, (min(batch, n),), device="cuda", generator=g)
-            cache.train_step(x[idx].contiguous(), y[idx].contiguous())
-        return n
+        def train():
+            b = ns.walks(tp, su.train_pairs, N.NC_COUPLED, su.depth, stream_id=7, seed=seed)
+            x, y = ns.training_pairs(b)
+            g = torch.Generator(device="cuda").manual_seed(seed)
+            n = x.shape[0]
+            for _ in range(su.train_steps):
+                idx = torch.randint(0, n, (min(su.batch, n),), device="cuda", generator=g)
+                cache.train_step(x[idx].contiguous(), y[idx].contiguous())
+            return n
 
-    ntrain, t_train = timed(train)
-    out["train"] = {"pairs": int(ntrain), "steps": train_steps, "batch": batch, "seconds": t_train}
-    log(f"train pairs={ntrain} steps={train_steps} t={t_train:.3f}s")
+        ntrain, t_train = timed(train)
+        res["train_label_walks"] = int(ntrain)
+        res["train_seconds"] = t_train
+        log(f"[{su.name}] train labels={ntrain} steps={su.train_steps} t={t_train:.3f}s")
 
-    # cache quality at walk positions: compare C(X_m) with the exact u(X_m)
-    b = ns.walks(tp[:: max(1, len(points) // 4096)].contiguous(), 4, N.NC_PREFIX, depth, stream_id=9, seed=seed)
-    sel = b.need.bool()
-    xq = b.xm[sel].contiguous()
-    if xq.shape[0] > 0:
-        from . import problems as P
-        cq = cache.infer(xq).cpu().numpy()
-        uq = P.charges_solution_f64(boundary.charges, xq.cpu().numpy()) if boundary.charges is not None else None
-        if uq is not None:
-            out["cache_rmse_at_xm"] = _rmse(cq, uq)
-            log(f"cache rmse at X_m = {out['cache_rmse_at_xm']:.4g}")
+        # cache quality at held-out walk positions X_m (exact u there)
+        b = ns.walks(tp[:: max(1, len(points) // 4096)].contiguous(), 4, N.NC_PREFIX, su.depth, stream_id=9,
+                     seed=seed)
+        sel = b.need.bool()
+        xq = b.xm[sel].contiguous()
+        if xq.shape[0] > 0 and boundary.charges is not None:
+            from . import problems as P
+            cq = cache.infer(xq).cpu().numpy()
+            res["cache_rmse_at_xm"] = _rmse(cq, P.charges_solution_f64(boundary.charges, xq.cpu().numpy()))
+            log(f"[{su.name}] cache rmse at X_m = {res['cache_rmse_at_xm']:.4g}")
 
-    # ---------------------------------------------------------------- NC
-    ncr = []
-    for wpp in nc_wpp:
-        (st, _), dt = timed(lambda: ns.nc(cache, tp, wpp, depth, stream_id=1, seed=seed + wpp))
-        r = _rmse(st.mean.cpu().numpy(), truth)
-        ncr.append({"wpp": wpp, "seconds": dt, "seconds_with_training": dt + t_train, "rmse": r})
-        log(f"nc wpp={wpp} t={dt:.3f}s (+train {t_train:.3f}s) rmse={r:.4g}")
-    out["nc"] = ncr
-    ok = [p["seconds_with_training"] for p in ncr if p["rmse"] <= tau]
-    out["nc_time_to_rmse"] = min(ok) if ok else None
-
-    # ---------------------------------------------------------------- 2LMC
-    tl = []
-    for nc_, nr in lmc:
-        (res, dt) = timed(lambda: ns.two_level(cache, tp, nc_, nr, depth, seed=seed + nc_ + 1000 * nr))
-        est, co, re, _ = res
-        r = _rmse(est.cpu().numpy(), truth)
-        vc = float((co.m2 / (co.n - 1)).mean().item())
-        vr = float((re.m2 / (re.n - 1)).mean().item())
-        tl.append({"coarse_wpp": nc_, "pairs": nr, "seconds": dt, "seconds_with_training": dt + t_train, "rmse": r,
-                   "coarse_var": vc, "residual_var": vr})
-        log(f"2lmc coarse={nc_} pairs={nr} t={dt:.3f}s (+train) rmse={r:.4g} Vc={vc:.4g} Vr={vr:.4g}")
-    out["nc_2lmc"] = tl
-    ok = [p["seconds_with_training"] for p in tl if p["rmse"] <= tau]
-    out["nc_2lmc_time_to_rmse"] = min(ok) if ok else None
-    pt = out["plain_time_to_rmse"]
+        ncr = []
+        for wpp in su.nc_wpp:
+            (st, _), dt = timed(lambda: ns.nc(cache, tp, wpp, su.depth, stream_id=1, seed=seed + wpp))
+            r = _rmse(st.mean.cpu().numpy(), truth)
+            ncr.append({"wpp": wpp, "seconds": dt, "seconds_with_training": dt + t_train, "rmse": r})
+            log(f"[{su.name}] nc wpp={wpp} t={dt:.3f}s (+train {t_train:.3f}s) rmse={r:.4g}")
+        res["nc"] = ncr
+        tl = []
+        for nc_, nr in su.lmc:
+            (r2, dt) = timed(lambda: ns.two_level(cache, tp, nc_, nr, su.depth, seed=seed + nc_ + 1000 * nr))
+            est, co, re, _ = r2
+            r = _rmse(est.cpu().numpy(), truth)
+            vc = float((co.m2 / (co.n - 1)).mean().item())
+            vr = float((re.m2 / (re.n - 1)).mean().item())
+            tl.append({"coarse_wpp": nc_, "pairs": nr, "seconds": dt, "seconds_with_training": dt + t_train,
+                       "rmse": r, "coarse_var": vc, "residual_var": vr})
+            log(f"[{su.name}] 2lmc coarse={nc_} pairs={nr} t={dt:.3f}s (+train) rmse={r:.4g} Vc={vc:.4g} "
+                f"Vr={vr:.4g}")
+        res["nc_2lmc"] = tl
+        for k, runs in (("nc", ncr), ("nc_2lmc", tl)):
+            ok = [p["seconds_with_training"] for p in runs if p["rmse"] <= tau]
+            v = min(ok) if ok else None
+            res[f"{k}_time_to_rmse"] = v
+            res[f"{k}_speedup_vs_plain"] = (pt / v) if (pt and v) else None
+            if v is not None and (k not in best or v < best[k][0]):
+                best[k] = (v, su.name)
+        out["setups"].append(res)
+        del cache
     for k in ("nc", "nc_2lmc"):
-        v = out[f"{k}_time_to_rmse"]
-        out[f"{k}_speedup_vs_plain"] = (pt / v) if (pt and v) else None
+        v = best.get(k)
+        out[f"best_{k}_time_to_rmse"] = v[0] if v else None
+        out[f"best_{k}_setup"] = v[1] if v else None
+        out[f"best_{k}_speedup_vs_plain"] = (pt / v[0]) if (pt and v) else None
     return out

@@ -99,16 +99,19 @@ def test_argument_errors(eng):
     assert b"stacks" in lib.n2wos_last_error()
     v, t = eng.uv_sphere(8, 16, 1, [0.1])
     n = C.c_int64()
-    rc = lib.n2wos_bvh_build_host(v.ctypes.data, len(v), t.ctypes.data, len(t), 0, None, C.byref(n), None, None)
+    rc = lib.n2wos_bvh_build_host(v.ctypes.data, len(v), t.ctypes.data, len(t), 0, None, None, C.byref(n), None, None)
     assert rc == eng._lib.ERR_INVALID
-    rc = lib.n2wos_bvh_build_host(v.ctypes.data, len(v), t.ctypes.data, len(t), 32, None, C.byref(n), None, None)
+    rc = lib.n2wos_bvh_build_host(v.ctypes.data, len(v), t.ctypes.data, len(t), 32, None, None, C.byref(n), None, None)
     assert rc == eng._lib.ERR_INVALID
     bad = t.copy()
     bad[0, 0] = len(v)
-    rc = lib.n2wos_bvh_build_host(v.ctypes.data, len(v), bad.ctypes.data, len(t), 8, None, C.byref(n), None, None)
+    rc = lib.n2wos_bvh_build_host(v.ctypes.data, len(v), bad.ctypes.data, len(t), 8, None, None, C.byref(n), None, None)
     assert rc == eng._lib.ERR_INVALID and b"out-of-range" in lib.n2wos_last_error()
-    rc = lib.n2wos_bvh_build_host(v.ctypes.data, len(v), t.ctypes.data, 0, 8, None, C.byref(n), None, None)
+    rc = lib.n2wos_bvh_build_host(v.ctypes.data, len(v), t.ctypes.data, 0, 8, None, None, C.byref(n), None, None)
     assert rc == eng._lib.ERR_EMPTY
+    rc = lib.n2wos_bvh_build_host(v.ctypes.data, len(v), t.ctypes.data, len(t), 8, b"radix", None, C.byref(n), None,
+                                  None)
+    assert rc == eng._lib.ERR_UNSUPPORTED and b"radix" in lib.n2wos_last_error()
     with pytest.raises(eng.N2wosError):
         eng.uv_sphere(64, 4)
     rc = lib.n2wos_bvh_query_device(None, None, 10, None, None, None, None, 128, None)

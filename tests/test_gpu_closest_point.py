@@ -107,3 +107,24 @@ def test_query_errors(eng, torch_gpu, c1):
     assert e.value.code == eng._lib.ERR_NULL
     st = bvh.stats()
     assert st["triangle_count"] == len(t) and st["leaf_size"] == 8 and st["build_method"] == "median"
+
+
+@pytest.mark.parametrize("mesh,leaf", [("mesh2", 4), ("mesh2", 8), ("blob", 4)])
+def test_sah_build_bit_exact_vs_bruteforce(eng, orc, torch_gpu, mesh, leaf):
+    """The surface-area-heuristic tree changes only the visit order: results
+    stay equal to the brute force (uniform, near-surface and WoS-prefix
+    queries), and the stack never overflows."""
+    from n2wos_amd import problems as P
+    v, t = P.make_mesh(mesh)
+    bvh = eng.Bvh(v, t, leaf_size=leaf, build_method="sah")
+    assert bvh.stats()["build_method"] == "sah"
+    rng = np.random.default_rng(7)
+    interior = P.interior_points(bvh, v, 512, seed=8)
+    pts = np.concatenate([rng.uniform(-1.2, 1.2, (1536, 3)).astype(np.float32),
+                          P.surface_shell_queries(v, t, 512, seed=9),
+                          P.wos_prefix_queries(bvh, interior, 512, seed=10)]).astype(np.float32)
+    print("case: sah", mesh, leaf, len(pts), flush=True)
+    d2, cp, tri, ovf = bvh.query(pts)
+    d2b, cpb, trb = orc.bruteforce(v, t, pts, nthreads=16)
+    assert np.array_equal(bits(d2), bits(d2b)) and np.array_equal(tri, trb) and np.array_equal(bits(cp), bits(cpb))
+    assert ovf.sum() == 0

@@ -103,3 +103,46 @@ def test_exact_cache_residual_has_zero_mean(eng, torch_gpu, m1):
     z = z[np.isfinite(z) & (re.m2.cpu().numpy() > 0)]
     assert abs(z.mean()) < 5 / np.sqrt(len(z))
     assert np.mean(np.abs(z) > 3) < 0.02
+
+
+def test_trained_cache_gain_over_plain_and_untrained(eng, torch_gpu, m1):
+    """What the trained network gains: the cache is trained on the GPU from
+    the solver's own coupled walks (depth 1, 32 per point, 100 Adam steps),
+    then the neural-cache estimate S_1 + C(X_1) with 64 samples per point --
+    at most 64 walk steps per point -- is compared with plain WoS given MORE
+    walk steps per point (4 full walks) and with the same estimator on the
+    untrained network.  RMSE against the FP64 solution."""
+    from n2wos_amd import cache as K, nc as N, problems as P, wos as W
+    torch = torch_gpu
+    v, t, bvh, _ = m1
+    pts = P.interior_points(bvh, v, 4096, seed=12)
+    tp = torch.from_numpy(pts).cuda()
+    bc = P.charges_boundary(0)
+    truth = P.charges_solution_f64(bc.charges, pts)
+    wp = W.WalkParams(epsilon=1e-4, seed=31)
+    s = N.NcSolver(bvh, bc, wp)
+    cfg = K.CacheConfig.for_bounds(v.min(0), v.max(0), n_levels=8, seed=5)
+    trained, untrained = K.Cache(cfg), K.Cache(cfg)
+    print("case: cache gain (train depth 1, 32 pairs, 100 steps)", flush=True)
+    b = s.walks(tp, 32, N.NC_COUPLED, 1, stream_id=7, seed=41)
+    x, y = s.training_pairs(b)
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    for _ in range(100):
+        idx = torch.randint(0, x.shape[0], (1 << 16,), device="cuda", generator=gen)
+        trained.train_step(x[idx].contiguous(), y[idx].contiguous())
+
+    def rmse(st):
+        return float(np.sqrt(np.mean((st.mean.cpu().numpy() - truth) ** 2)))
+
+    nc_t, wb = s.nc(trained, tp, 64, 1, stream_id=1, seed=51)
+    nc_u, _ = s.nc(untrained, tp, 64, 1, stream_id=1, seed=51)
+    plain = W.Solver(bvh, bc, wp).solve(tp, 4, seed=61)
+    torch.cuda.synchronize()
+    nc_steps = float(wb.steps.sum().item()) / len(pts)
+    plain_steps = float(plain.steps.sum().item()) / len(pts)
+    r_t, r_u, r_p = rmse(nc_t), rmse(nc_u), rmse(plain)
+    print("rmse trained %.4g untrained %.4g plain %.4g; steps/point nc %.1f plain %.1f" %
+          (r_t, r_u, r_p, nc_steps, plain_steps), flush=True)
+    assert nc_steps <= 64 < plain_steps
+    assert r_t < 0.5 * r_p
+    assert r_t < 0.25 * r_u

@@ -37,10 +37,11 @@ def emu():
     return E
 
 
-def _case(eng, orc, emu, mesh, npts, wpp, poisson, weights=(10, 10, 6), refill=8, eps=1e-3, seed=77):
+def _case(eng, orc, emu, mesh, npts, wpp, poisson, weights=(10, 10, 6), refill=8, eps=1e-3, seed=77,
+          build_method="median", leaf=8):
     from n2wos_amd import problems as P
     v, t = P.make_mesh(mesh)
-    nodes, recs, delta = eng.build_host(v, t)
+    nodes, recs, delta = eng.build_host(v, t, leaf_size=leaf, build_method=build_method)
     rng = np.random.default_rng(npts + wpp)
     d = rng.normal(size=(npts, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
@@ -86,3 +87,8 @@ def test_emulated_walker_poisson_small_units(eng, orc, emu):
 def test_emulated_walker_extreme_schedules(eng, orc, emu, weights, refill):
     """Any phase choice and refill threshold gives the same walks."""
     _case(eng, orc, emu, "mesh1", 3, 64, False, weights=weights, refill=refill)
+
+
+def test_emulated_walker_sah_tree(eng, orc, emu):
+    """Walks over a surface-area-heuristic tree (leaf 4) equal the oracle's."""
+    _case(eng, orc, emu, "mesh2", 5, 64, False, weights=(10, 10, 10), refill=2, eps=1e-4, build_method="sah", leaf=4)
