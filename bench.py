@@ -165,7 +165,7 @@ def wos_problem1(eng, torch, P, W, cpu: bool, threads: int) -> dict:
 def poisson_problem3(eng, torch, P, W) -> dict:
     t0 = time.perf_counter()
     v, t = P.make_mesh("mesh3")
-    bvh = eng.Bvh(v, t, leaf_size=4)
+    bvh = eng.Bvh(v, t, leaf_size=4, build_method="sah")
     build_s = time.perf_counter() - t0
     pts = P.interior_points(bvh, v, 262144, seed=0)
     bc = P.poisson_boundary()

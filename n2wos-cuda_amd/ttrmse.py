@@ -38,12 +38,15 @@ class CacheSetup:
 
 # The learned-cache configuration of the benchmark problem (16 levels x 2
 # features, T = 2^19, N_min 16, growth 1.5; 4 x 64 ReLU MLP; depth 4; 32
-# label walks per point; 16 coupled pairs per point for 2LMC) and the best
-# setup found on the same problem (depth 1, 8 levels, 100 Adam steps).
+# label walks per point; 16 coupled pairs per point for 2LMC) and two
+# shallower setups found on the same problem (depth 1; 8 or 4 levels, whose
+# coarser grids generalise better from single-walk labels; 100 Adam steps).
 DEFAULT_SETUPS = (
     CacheSetup("depth4_levels16", depth=4, train_steps=200, nc_wpp=(64, 256), lmc=((256, 16), (512, 16))),
     CacheSetup("depth1_levels8", depth=1, cache_kw={"n_levels": 8}, train_steps=100, nc_wpp=(128, 192, 256),
                lmc=()),
+    CacheSetup("depth1_levels4", depth=1, cache_kw={"n_levels": 4}, train_steps=100, nc_wpp=(96, 128, 192, 256),
+               lmc=((128, 16),)),
 )
 
 

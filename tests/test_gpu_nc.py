@@ -106,24 +106,26 @@ def test_exact_cache_residual_has_zero_mean(eng, torch_gpu, m1):
 
 
 def test_trained_cache_gain_over_plain_and_untrained(eng, torch_gpu, m1):
-    """What the trained network gains: the cache is trained on the GPU from
-    the solver's own coupled walks (depth 1, 32 per point, 100 Adam steps),
-    then the neural-cache estimate S_1 + C(X_1) with 64 samples per point --
-    at most 64 walk steps per point -- is compared with plain WoS given MORE
-    walk steps per point (4 full walks) and with the same estimator on the
-    untrained network.  RMSE against the FP64 solution."""
+    """What the trained network gains: a 4-level cache is trained on the GPU
+    from the solver's own coupled walks (depth 1, 32 per point over 16,384
+    interior points, 100 Adam steps), then the neural-cache estimate
+    S_1 + C(X_1) with 64 samples per point -- at most 64 walk steps per
+    point -- is compared with plain WoS given MORE walk steps per point (4
+    full walks, ~98 steps) and with the same estimator on the untrained
+    network.  RMSE against the FP64 solution (measured: 0.014 vs 0.085 plain,
+    1.08 untrained)."""
     from n2wos_amd import cache as K, nc as N, problems as P, wos as W
     torch = torch_gpu
     v, t, bvh, _ = m1
-    pts = P.interior_points(bvh, v, 4096, seed=12)
+    pts = P.interior_points(bvh, v, 16384, seed=12)
     tp = torch.from_numpy(pts).cuda()
     bc = P.charges_boundary(0)
     truth = P.charges_solution_f64(bc.charges, pts)
     wp = W.WalkParams(epsilon=1e-4, seed=31)
     s = N.NcSolver(bvh, bc, wp)
-    cfg = K.CacheConfig.for_bounds(v.min(0), v.max(0), n_levels=8, seed=5)
+    cfg = K.CacheConfig.for_bounds(v.min(0), v.max(0), n_levels=4, seed=5)
     trained, untrained = K.Cache(cfg), K.Cache(cfg)
-    print("case: cache gain (train depth 1, 32 pairs, 100 steps)", flush=True)
+    print("case: cache gain (4 levels, train depth 1, 32 pairs, 100 steps)", flush=True)
     b = s.walks(tp, 32, N.NC_COUPLED, 1, stream_id=7, seed=41)
     x, y = s.training_pairs(b)
     gen = torch.Generator(device="cuda").manual_seed(3)
@@ -144,5 +146,5 @@ def test_trained_cache_gain_over_plain_and_untrained(eng, torch_gpu, m1):
     print("rmse trained %.4g untrained %.4g plain %.4g; steps/point nc %.1f plain %.1f" %
           (r_t, r_u, r_p, nc_steps, plain_steps), flush=True)
     assert nc_steps <= 64 < plain_steps
-    assert r_t < 0.5 * r_p
-    assert r_t < 0.25 * r_u
+    assert r_t < 0.4 * r_p
+    assert r_t < 0.1 * r_u
