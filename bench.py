@@ -129,6 +129,49 @@ def closest_point_stress(eng, torch, P, n: int = 262144) -> dict:
     return out
 
 
+def cpu_closest_point(P, threads: int, gpu_qps: float, sample: int = 2048) -> dict:
+    """The reference's CPU path for closest points -- closest_point_bruteforce
+    (src/geometry/closest_point.cpp:50-68), single-threaded as in its probe
+    (src/tools/probe_cuda_bvh.cpp:153-167) -- timed on a bounded sample of the
+    problem-1 queries, beside the oracle's brute force and its binary-BVH
+    traversal on 1 and on all host threads.  Every CPU result is checked
+    equal to the GPU's on the sample."""
+    import oracle
+    oracle.build()
+    v1, t1 = P.make_mesh("mesh1")
+    q = P.uniform_box_queries(65536, seed=0)[:sample]
+    out = {"queries": sample, "of": 65536, "threads_all": threads, "gpu_queries_per_s": gpu_qps}
+
+    def rate(fn, n):
+        t0 = time.perf_counter()
+        r = fn()
+        return n / (time.perf_counter() - t0), r
+
+    d2o, _, trio = oracle.bruteforce(v1, t1, q, 1)
+    if oracle.ref_available():
+        R = oracle.ref()
+        d2 = np.zeros(sample, np.float32)
+        cp = np.zeros((sample, 3), np.float32)
+        tri = np.zeros(sample, np.int32)
+        qps, _ = rate(lambda: R.ref_bruteforce(np.ascontiguousarray(v1), len(v1), np.ascontiguousarray(t1),
+                                               len(t1), q, sample, d2, cp, tri), sample)
+        out["reference_bruteforce_1_thread"] = {"queries_per_s": qps, "kind": "reference",
+                                                "equal_to_oracle": bool(np.array_equal(tri, trio))}
+    qps1, _ = rate(lambda: oracle.bruteforce(v1, t1, q, 1), sample)
+    qpsn, _ = rate(lambda: oracle.bruteforce(v1, t1, q, threads), sample)
+    out["oracle_bruteforce"] = {"queries_per_s_1_thread": qps1, "queries_per_s_all_threads": qpsn, "kind": "port"}
+    b2 = oracle.Bvh2(v1, t1)
+    big = P.uniform_box_queries(65536, seed=0)
+    qb1, r1 = rate(lambda: b2.query(big, 1), 65536)
+    qbn, _ = rate(lambda: b2.query(big, threads), 65536)
+    out["oracle_bvh2"] = {"queries": 65536, "queries_per_s_1_thread": qb1, "queries_per_s_all_threads": qbn,
+                          "kind": "port", "equal_to_bruteforce_on_sample": bool(np.array_equal(r1[2][:sample], trio))}
+    log(f"cpu closest point: reference brute force "
+        f"{out.get('reference_bruteforce_1_thread', {}).get('queries_per_s', float('nan')):.0f} q/s, "
+        f"oracle bvh {qbn / 1e6:.2f} M q/s on {threads} threads")
+    return out
+
+
 def wos_problem1(eng, torch, P, W, cpu: bool, threads: int) -> dict:
     v, t = P.make_mesh("mesh1")
     bvh = eng.Bvh(v, t)
@@ -347,6 +390,8 @@ def main():
     threads = cpu_threads()
     extra = line["extra"]
     extra["closest_point_stress"] = closest_point_stress(eng, torch, P)
+    if not args.no_cpu:
+        extra["closest_point_problem1_cpu"] = cpu_closest_point(P, threads, 65536 / (cpq_ms * 1e-3))
     extra["wos_problem1"] = wos_problem1(eng, torch, P, W, not args.no_cpu, threads)
     extra["cache"] = cache_rates(eng, torch)
     if not args.quick:

@@ -68,7 +68,9 @@ int n2wos_abi_version(void);
 /* Launch tuning for benchmarks (process-wide): "ring" = LDS stack entries
  * per lane of the walker (8 or 16, default 16); "refill_min" = idle lanes
  * that trigger a refill (1..64, default 2); "weight_node" / "weight_tri" /
- * "weight_update" = per-lane weights of the walker's phase choice (10 each). */
+ * "weight_update" = per-lane weights of the walker's phase choice (10 each);
+ * "merge" = 1 serves node visits and triangle tests in one iteration of the
+ * walker (0 = one phase per iteration, the default). */
 int n2wos_set_tuning(const char* key, int value);
 /* Device name and runtime version of the current HIP device, for reports. */
 int n2wos_device_summary(char* buf, int buf_len);
@@ -236,7 +238,13 @@ int n2wos_wos_solve(const n2wos_bvh* bvh, const n2wos_boundary* boundary, const 
  *     to the boundary; d_full = the full walk's value (shared prefix).
  * Outputs have n_points * walks_per_point entries (d_xm three floats each);
  * d_steps receives per-point step totals.  Workspace: n2wos_nc_workspace_size. */
-enum { N2WOS_NC_PREFIX = 1, N2WOS_NC_COUPLED = 2 };
+/* N2WOS_NC_ANTITHETIC may be OR-ed into the mode: walks 2j and 2j+1 of a
+ * point then take mirrored first steps (the odd walk uses its even partner's
+ * step-0 draw with the direction, and the source-sample offset, negated).
+ * Each walk keeps its marginal law, so every estimator stays as it was in
+ * expectation; the pair mean cancels the odd part of the step-0 variation.
+ * (No counterpart in the reference: a variance-reduction option.) */
+enum { N2WOS_NC_PREFIX = 1, N2WOS_NC_COUPLED = 2, N2WOS_NC_ANTITHETIC = 0x100 };
 int64_t n2wos_nc_workspace_size(int n_points, int walks_per_point);
 int n2wos_nc_walks(const n2wos_bvh* bvh, const n2wos_boundary* boundary, const n2wos_walk_params* params,
                    const float* d_points, int n_points, int walks_per_point, uint32_t point_offset, int mode,

@@ -67,6 +67,70 @@ N2_HD V3 closest_on_triangle(V3 p, V3 a, V3 b, V3 c) {
   return a + ab * v + ac * w;
 }
 
+// The same closest point, computed without branches for SIMD execution: all
+// six dot products and the three barycentric numerators are evaluated, the
+// Voronoi region is chosen by the same tests in the same order as above, and
+// the one division of that region (1 / (va + vb + vc) for the face) is done
+// once on the selected operands.  Every value returned is produced by the same
+// IEEE operations on the same operands as closest_on_triangle, so the two are
+// bit-identical; a wave whose lanes fall in different regions then executes
+// one division instead of up to four.
+N2_HD V3 closest_on_triangle_flat(V3 p, V3 a, V3 b, V3 c) {
+  const V3 ab = b - a;
+  const V3 ac = c - a;
+  const V3 ap = p - a;
+  const float d1 = dot(ab, ap);
+  const float d2 = dot(ac, ap);
+  const V3 bp = p - b;
+  const float d3 = dot(ab, bp);
+  const float d4 = dot(ac, bp);
+  const float vc = d1 * d4 - d3 * d2;
+  const V3 cp = p - c;
+  const float d5 = dot(ab, cp);
+  const float d6 = dot(ac, cp);
+  const float vb = d5 * d2 - d1 * d6;
+  const float va = d3 * d6 - d5 * d4;
+  const float e43 = d4 - d3;
+  const float e56 = d5 - d6;
+  const bool rA = d1 <= 0.0f && d2 <= 0.0f;
+  const bool rB = !rA && d3 >= 0.0f && d4 <= d3;
+  const bool vtxAB = rA || rB;
+  const bool rAB = !vtxAB && vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f;
+  const bool rC = !vtxAB && !rAB && d6 >= 0.0f && d5 <= d6;
+  const bool done4 = vtxAB || rAB || rC;
+  const bool rAC = !done4 && vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f;
+  const bool rBC = !done4 && !rAC && va <= 0.0f && e43 >= 0.0f && e56 >= 0.0f;
+  const bool rFace = !done4 && !rAC && !rBC;
+  // the region's division
+  const float num = rAB ? d1 : (rAC ? d2 : (rBC ? e43 : 1.0f));
+  const float den = rAB ? (d1 - d3) : (rAC ? (d2 - d6) : (rBC ? e43 + e56 : (va + vb + vc)));
+  const float q = num / den;
+  // edge: base + dir * q; face: (a + ab * (vb q)) + ac * (vc q).  Selects
+  // are written per component so they stay register selects.
+  const V3 cb = c - b;
+  const float s = rFace ? vb * q : q;
+  const float w = vc * q;
+  const int axis_src = rAC ? 1 : (rBC ? 2 : 0);
+  V3 r;
+#define N2_FLAT_AXIS(X)                                                              \
+  {                                                                                \
+    const float base = rBC ? b.X : a.X;                                            \
+    const float dir = axis_src == 1 ? ac.X : (axis_src == 2 ? cb.X : ab.X);        \
+    const float e1 = base + dir * s;                                               \
+    const float face = e1 + ac.X * w;                                              \
+    float v = rFace ? face : e1;                                                   \
+    v = rA ? a.X : v;                                                              \
+    v = rB ? b.X : v;                                                              \
+    v = rC ? c.X : v;                                                              \
+    r.X = v;                                                                       \
+  }
+  N2_FLAT_AXIS(x)
+  N2_FLAT_AXIS(y)
+  N2_FLAT_AXIS(z)
+#undef N2_FLAT_AXIS
+  return r;
+}
+
 // Squared distance from p to an axis-aligned box (0 inside).  An empty box
 // (lo = +inf, hi = -inf) yields +inf.
 N2_HD float box_dist2(V3 p, float lox, float loy, float loz, float hix, float hiy, float hiz) {

@@ -53,6 +53,12 @@ __host__ __device__ __forceinline__ void chan_merge(double na, double ma, double
 #endif
 constexpr int kWaves = N2_WAVES_PER_BLOCK;  // waves per block
 constexpr int kBlock = 64 * kWaves;
+// Waves per SIMD the register allocation must allow (the LDS ring of 16
+// entries per lane allows about 4 waves per SIMD, 8 entries about 6).
+#ifndef N2_WALKER_WAVES_PER_SIMD
+#define N2_WALKER_WAVES_PER_SIMD 4
+#endif
+
 
 // Walk modes: plain walks; prefix walks that stop at depth m and leave the
 // cache query point X_m (neural-cache WoS); coupled walks that record the
@@ -72,8 +78,8 @@ constexpr int kUpdate = 3;  // query finished: take the walk step (or end the wa
 // for paths they do not take, and a lane whose query ends starts its next
 // query without waiting for the rest of the wave.  Each lane's own sequence of
 // node visits, triangle tests and walk steps is exactly the sequential one.
-template <int CAP, int S, bool POISSON, bool COUNT>
-__global__ void __launch_bounds__(kBlock) k_wos_persistent(WosKernelArgs a) {
+template <int CAP, int S, bool POISSON, bool COUNT, bool MERGE = false>
+__global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persistent(WosKernelArgs a) {
   __shared__ uint32_t s_ref[CAP * kBlock];
   __shared__ float s_d2[CAP * kBlock];
   __shared__ float s_val[kWaves][S][64];
@@ -166,6 +172,59 @@ __global__ void __launch_bounds__(kBlock) k_wos_persistent(WosKernelArgs a) {
     top = (top + 1) & (CAP - 1);                     \
   } while (0)
 
+  // one node visit: the node's line is (lox, loy, loz, hix, hiy, hiz, child refs)
+#define N2_NODE_VISIT(lox, loy, loz, hix, hiy, hiz, chf)                          \
+  do {                                                                             \
+    if (COUNT) ++n_nodes;                                                          \
+    float k0 = box_dist2(p, lox.x, loy.x, loz.x, hix.x, hiy.x, hiz.x);             \
+    float k1 = box_dist2(p, lox.y, loy.y, loz.y, hix.y, hiy.y, hiz.y);             \
+    float k2 = box_dist2(p, lox.z, loy.z, loz.z, hix.z, hiy.z, hiz.z);             \
+    float k3 = box_dist2(p, lox.w, loy.w, loz.w, hix.w, hiy.w, hiz.w);             \
+    const int nvalid = (k0 <= thr) + (k1 <= thr) + (k2 <= thr) + (k3 <= thr);      \
+    if (nvalid == 0) {                                                             \
+      N2_POP_NEXT();                                                               \
+    } else {                                                                       \
+      uint32_t r0 = static_cast<uint32_t>(__float_as_int(chf.x));                  \
+      uint32_t r1 = static_cast<uint32_t>(__float_as_int(chf.y));                  \
+      uint32_t r2 = static_cast<uint32_t>(__float_as_int(chf.z));                  \
+      uint32_t r3 = static_cast<uint32_t>(__float_as_int(chf.w));                  \
+      int s0 = 0, s1 = 1, s2 = 2, s3 = 3;                                          \
+      k0 = k0 <= thr ? k0 : INFINITY;                                              \
+      k1 = k1 <= thr ? k1 : INFINITY;                                              \
+      k2 = k2 <= thr ? k2 : INFINITY;                                              \
+      k3 = k3 <= thr ? k3 : INFINITY;                                              \
+      cswap(k0, r0, s0, k1, r1, s1);                                               \
+      cswap(k2, r2, s2, k3, r3, s3);                                               \
+      cswap(k0, r0, s0, k2, r2, s2);                                               \
+      cswap(k1, r1, s1, k3, r3, s3);                                               \
+      cswap(k1, r1, s1, k2, r2, s2);                                               \
+      if (nvalid >= 4) N2_PUSH(r3, k3);                                            \
+      if (nvalid >= 3) N2_PUSH(r2, k2);                                            \
+      if (nvalid >= 2) N2_PUSH(r1, k1);                                            \
+      N2_ENTER(r0);                                                                \
+    }                                                                              \
+  } while (0)
+  // one triangle record (a, b, c, original index) of the current leaf.  The
+  // exact test runs for every record: the wave is VALU-bound, and a per-lane
+  // box pre-test saves nothing unless every served lane rejects.
+#define N2_TRI_TEST(t0, t1, t2)                                                    \
+  do {                                                                             \
+    if (COUNT) ++n_tris;                                                           \
+    const V3 ta = v3(t0.x, t0.y, t0.z), tb = v3(t0.w, t1.x, t1.y), tc = v3(t1.z, t1.w, t2.x); \
+    const int id = __float_as_int(t2.y);                                           \
+    const V3 q = closest_on_triangle_flat(p, ta, tb, tc);                          \
+    const V3 e = q - p;                                                            \
+    const float d2 = dot(e, e);                                                    \
+    if (d2 < bd2 || (d2 == bd2 && id < bid)) {                                     \
+      bd2 = d2;                                                                    \
+      bcp = q;                                                                     \
+      bid = id;                                                                    \
+      thr = fminf(thr, prune_threshold(d2, a.delta));                              \
+    }                                                                              \
+    ++ti;                                                                          \
+    if (ti == te) N2_POP_NEXT();                                                   \
+  } while (0)
+
   while (true) {
     N2_EMU_LOOP_HOOK();
     // ---------------------------------------------------------- refill
@@ -249,7 +308,8 @@ __global__ void __launch_bounds__(kBlock) k_wos_persistent(WosKernelArgs a) {
     const int sU = __popcll(mU) * a.weight_update;
     N2_EMU_TRACE(sN, sT, sU);
 
-    if (sU >= sN && sU >= sT) {
+    // the walk step is served when its lanes outweigh the traversal's
+    if (MERGE ? (sU >= sN + sT) : (sU >= sN && sU >= sT)) {
       // ------------------------------------------------------ walk step
       bool unit_done = false;
       if (phase == kUpdate) {
@@ -303,15 +363,22 @@ __global__ void __launch_bounds__(kBlock) k_wos_persistent(WosKernelArgs a) {
               a.nc_xm[3 * sidx + 1] = p.y;
               a.nc_xm[3 * sidx + 2] = p.z;
             }
-            const U4 rr = philox4x32_10(U4{static_cast<uint32_t>(step), ctr_walk, ctr_point, 2u * a.stream},
+            // antithetic pairs: the odd walk of a pair takes its first step
+            // along the negated direction drawn for its even partner
+            const bool anti = a.antithetic && step == 0 && (walk & 1);
+            const uint32_t cw = anti ? ctr_walk - 1u : ctr_walk;
+            const float sgn = anti ? -1.0f : 1.0f;
+            const U4 rr = philox4x32_10(U4{static_cast<uint32_t>(step), cw, ctr_point, 2u * a.stream},
                                         a.seed_lo, a.seed_hi);
-            const V3 dir = sphere_dir(rr.x, rr.y);
+            V3 dir = sphere_dir(rr.x, rr.y);
+            dir = v3(sgn * dir.x, sgn * dir.y, sgn * dir.z);
             const float R = a.step_scale * d;
             if (POISSON) {
-              const U4 t = philox4x32_10(U4{static_cast<uint32_t>(step), ctr_walk, ctr_point, 2u * a.stream + 1u},
+              const U4 t = philox4x32_10(U4{static_cast<uint32_t>(step), cw, ctr_point, 2u * a.stream + 1u},
                                          a.seed_lo, a.seed_hi);
               const float rho = median3(u01(rr.z), u01(rr.w), u01(t.x));
-              const V3 e = sphere_dir(t.y, t.z);
+              V3 e = sphere_dir(t.y, t.z);
+              e = v3(sgn * e.x, sgn * e.y, sgn * e.z);
               const float Rr = R * rho;
               const V3 y = v3(p.x + Rr * e.x, p.y + Rr * e.y, p.z + Rr * e.z);
               src = src + (R * R) * poisson_source(y) / 6.0f;
@@ -366,71 +433,51 @@ __global__ void __launch_bounds__(kBlock) k_wos_persistent(WosKernelArgs a) {
         }
         wave_sync();
       }
+    } else if (MERGE) {
+      // ------------------------------------- node visits and triangle tests
+      // together: node lanes and triangle lanes issue their loads at once,
+      // so both wait for one memory latency per iteration
+      const bool isN = phase == kNode, isT = phase == kTri;
+      float4 m0, m1, m2, m3, m4, m5, m6;
+      if (isN || isT) {
+        const float4* mp = isN ? reinterpret_cast<const float4*>(a.nodes + cur)
+                               : reinterpret_cast<const float4*>(a.tris + ti);
+        m0 = mp[0];
+        m1 = mp[1];
+        m2 = mp[2];
+        if (isN) {
+          m3 = mp[3];
+          m4 = mp[4];
+          m5 = mp[5];
+          m6 = mp[6];
+        }
+      }
+      if (isN) {
+        N2_NODE_VISIT(m0, m1, m2, m3, m4, m5, m6);
+      } else if (isT) {
+        N2_TRI_TEST(m0, m1, m2);
+      }
     } else if (sN >= sT) {
       // ------------------------------------------------------ one node visit
       if (phase == kNode) {
         const float4* np = reinterpret_cast<const float4*>(a.nodes + cur);
-        const float4 lox = np[0], loy = np[1], loz = np[2];
-        const float4 hix = np[3], hiy = np[4], hiz = np[5];
-        const uint4 ch = reinterpret_cast<const uint4*>(np)[6];
-        if (COUNT) ++n_nodes;
-        float k0 = box_dist2(p, lox.x, loy.x, loz.x, hix.x, hiy.x, hiz.x);
-        float k1 = box_dist2(p, lox.y, loy.y, loz.y, hix.y, hiy.y, hiz.y);
-        float k2 = box_dist2(p, lox.z, loy.z, loz.z, hix.z, hiy.z, hiz.z);
-        float k3 = box_dist2(p, lox.w, loy.w, loz.w, hix.w, hiy.w, hiz.w);
-        const int nvalid = (k0 <= thr) + (k1 <= thr) + (k2 <= thr) + (k3 <= thr);
-        if (nvalid == 0) {
-          N2_POP_NEXT();
-        } else {
-          uint32_t r0 = ch.x, r1 = ch.y, r2 = ch.z, r3 = ch.w;
-          int s0 = 0, s1 = 1, s2 = 2, s3 = 3;
-          k0 = k0 <= thr ? k0 : INFINITY;
-          k1 = k1 <= thr ? k1 : INFINITY;
-          k2 = k2 <= thr ? k2 : INFINITY;
-          k3 = k3 <= thr ? k3 : INFINITY;
-          cswap(k0, r0, s0, k1, r1, s1);
-          cswap(k2, r2, s2, k3, r3, s3);
-          cswap(k0, r0, s0, k2, r2, s2);
-          cswap(k1, r1, s1, k3, r3, s3);
-          cswap(k1, r1, s1, k2, r2, s2);
-          if (nvalid >= 4) N2_PUSH(r3, k3);
-          if (nvalid >= 3) N2_PUSH(r2, k2);
-          if (nvalid >= 2) N2_PUSH(r1, k1);
-          N2_ENTER(r0);
-        }
+        const float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3], n4 = np[4], n5 = np[5], n6 = np[6];
+        N2_NODE_VISIT(n0, n1, n2, n3, n4, n5, n6);
       }
     } else {
       // ------------------------------------------------------ one triangle
       if (phase == kTri) {
         const float4* tp = reinterpret_cast<const float4*>(a.tris + ti);
         const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-        if (COUNT) ++n_tris;
-        // the triangle's own box bounds its distance from below: skip the
-        // exact test when even the box is beyond the pruning threshold
-        const float bxl = fminf(fminf(t0.x, t0.w), t1.z), bxh = fmaxf(fmaxf(t0.x, t0.w), t1.z);
-        const float byl = fminf(fminf(t0.y, t1.x), t1.w), byh = fmaxf(fmaxf(t0.y, t1.x), t1.w);
-        const float bzl = fminf(fminf(t0.z, t1.y), t2.x), bzh = fmaxf(fmaxf(t0.z, t1.y), t2.x);
-        if (box_dist2(p, bxl, byl, bzl, bxh, byh, bzh) <= thr) {
-          const V3 ta = v3(t0.x, t0.y, t0.z), tb = v3(t0.w, t1.x, t1.y), tc = v3(t1.z, t1.w, t2.x);
-          const int id = __float_as_int(t2.y);
-          const V3 q = closest_on_triangle(p, ta, tb, tc);
-          const V3 e = q - p;
-          const float d2 = dot(e, e);
-          if (d2 < bd2 || (d2 == bd2 && id < bid)) {
-            bd2 = d2;
-            bcp = q;
-            bid = id;
-            thr = fminf(thr, prune_threshold(d2, a.delta));
-          }
-        }
-        ++ti;
-        if (ti == te) N2_POP_NEXT();
+        N2_TRI_TEST(t0, t1, t2);
       }
     }
   }
 #undef N2_ENTER
 #undef N2_POP_NEXT
 #undef N2_PUSH
+#undef N2_NODE_VISIT
+#undef N2_TRI_TEST
   if (COUNT) {
     atomicAdd(reinterpret_cast<unsigned long long*>(a.visits), static_cast<unsigned long long>(n_nodes));
     atomicAdd(reinterpret_cast<unsigned long long*>(a.visits) + 1, static_cast<unsigned long long>(n_tris));

@@ -78,21 +78,38 @@ __global__ void k_point_merge(int n_points, int units_per_point, int walks_per_p
 
 __global__ void k_zero_counter(int* c) { *c = 0; }
 
-template <int CAP, int S, bool POISSON>
+template <int CAP, int S, bool POISSON, bool MERGE>
 void launch_wos(const WosKernelArgs& a, int grid, hipStream_t stream) {
   if (a.visits)
-    hipLaunchKernelGGL((k_wos_persistent<CAP, S, POISSON, true>), dim3(grid), dim3(kBlock), 0, stream, a);
+    hipLaunchKernelGGL((k_wos_persistent<CAP, S, POISSON, true, MERGE>), dim3(grid), dim3(kBlock), 0, stream, a);
   else
-    hipLaunchKernelGGL((k_wos_persistent<CAP, S, POISSON, false>), dim3(grid), dim3(kBlock), 0, stream, a);
+    hipLaunchKernelGGL((k_wos_persistent<CAP, S, POISSON, false, MERGE>), dim3(grid), dim3(kBlock), 0, stream, a);
 }
 
-template <int CAP, int S, bool POISSON>
+template <int CAP, int S, bool POISSON, bool MERGE>
 int max_blocks_per_cu() {
   int b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wos_persistent<CAP, S, POISSON, false>, kBlock, 0) !=
-      hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wos_persistent<CAP, S, POISSON, false, MERGE>, kBlock,
+                                                   0) != hipSuccess)
     return 1;
   return std::max(1, b);
+}
+
+// Kernel variant: LDS ring capacity ("ring": 16 or 8) and scheduler ("merge":
+// 1 serves node visits and triangle tests in the same iteration).
+template <int S, bool POISSON>
+int variant_blocks_per_cu(int cap, bool merge) {
+  if (cap == 16) return merge ? max_blocks_per_cu<16, S, POISSON, true>() : max_blocks_per_cu<16, S, POISSON, false>();
+  return merge ? max_blocks_per_cu<8, S, POISSON, true>() : max_blocks_per_cu<8, S, POISSON, false>();
+}
+
+template <int S, bool POISSON>
+void launch_variant(const WosKernelArgs& a, int grid, hipStream_t stream, int cap, bool merge) {
+  if (cap == 16) {
+    if (merge) launch_wos<16, S, POISSON, true>(a, grid, stream); else launch_wos<16, S, POISSON, false>(a, grid, stream);
+  } else {
+    if (merge) launch_wos<8, S, POISSON, true>(a, grid, stream); else launch_wos<8, S, POISSON, false>(a, grid, stream);
+  }
 }
 
 }  // namespace
@@ -127,11 +144,10 @@ int wos_full_grid(int source) {
   N2_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   constexpr int kSlots = 8;
   const bool poisson = source == N2WOS_SOURCE_POISSON_SINES;
-  int per_cu;
-  if (tuning_value("ring", 16) == 16)
-    per_cu = poisson ? max_blocks_per_cu<16, kSlots, true>() : max_blocks_per_cu<16, kSlots, false>();
-  else
-    per_cu = poisson ? max_blocks_per_cu<8, kSlots, true>() : max_blocks_per_cu<8, kSlots, false>();
+  const int cap = tuning_value("ring", 16) == 16 ? 16 : 8;
+  const bool merge = tuning_value("merge", 0) != 0;
+  const int per_cu = poisson ? variant_blocks_per_cu<kSlots, true>(cap, merge)
+                             : variant_blocks_per_cu<kSlots, false>(cap, merge);
   return cus * per_cu;
 }
 
@@ -195,16 +211,15 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
   a.weight_update = tuning_value("weight_update", 10);
   a.mode = r.mode;
   a.depth = r.depth;
+  a.antithetic = r.antithetic;
   a.nc_prefix = r.nc_prefix;
   a.nc_xm = r.nc_xm;
   a.nc_need = r.nc_need;
   a.nc_full = r.nc_full;
   if (r.ev_begin) N2_HIP(hipEventRecord(static_cast<hipEvent_t>(r.ev_begin), stream));
-  if (cap == 16) {
-    if (poisson) launch_wos<16, kSlots, true>(a, grid, stream); else launch_wos<16, kSlots, false>(a, grid, stream);
-  } else {
-    if (poisson) launch_wos<8, kSlots, true>(a, grid, stream); else launch_wos<8, kSlots, false>(a, grid, stream);
-  }
+  const bool merge = tuning_value("merge", 0) != 0;
+  if (poisson) launch_variant<kSlots, true>(a, grid, stream, cap == 16 ? 16 : 8, merge);
+  else launch_variant<kSlots, false>(a, grid, stream, cap == 16 ? 16 : 8, merge);
   if (r.ev_end) N2_HIP(hipEventRecord(static_cast<hipEvent_t>(r.ev_end), stream));
   N2_HIP(hipGetLastError());
   const int mb = 256;
@@ -315,6 +330,8 @@ extern "C" int n2wos_nc_walks(const n2wos_bvh* bvh, const n2wos_boundary* bounda
                               int mode, int depth_m, float* d_prefix, float* d_xm, uint8_t* d_need, float* d_full,
                               uint64_t* d_steps, void* d_workspace, int64_t workspace_bytes, void* stream) {
   return guarded([&]() -> int {
+    const int antithetic = (mode & N2WOS_NC_ANTITHETIC) ? 1 : 0;
+    mode &= ~N2WOS_NC_ANTITHETIC;
     N2_REQUIRE(mode == N2WOS_NC_PREFIX || mode == N2WOS_NC_COUPLED, N2WOS_ERR_INVALID, "mode must be prefix or coupled");
     N2_REQUIRE(depth_m >= 0, N2WOS_ERR_INVALID, "depth_m must be >= 0");
     N2_REQUIRE(bvh && boundary && wp, N2WOS_ERR_NULL, "NULL handle, boundary or parameters");
@@ -358,6 +375,7 @@ extern "C" int n2wos_nc_walks(const n2wos_bvh* bvh, const n2wos_boundary* bounda
     r.ev_end = wp->event_end;
     r.mode = mode == N2WOS_NC_PREFIX ? kModePrefix : kModeCoupled;
     r.depth = depth_m;
+    r.antithetic = antithetic;
     r.nc_prefix = d_prefix;
     r.nc_xm = d_xm;
     r.nc_need = d_need;

@@ -47,6 +47,7 @@ struct WosKernelArgs {
   int weight_update;
   int mode;               // kModePlain / kModePrefix / kModeCoupled
   int depth;              // m of the prefix / coupled modes
+  int antithetic;         // 1: walks 2j and 2j+1 take mirrored first steps
   float* nc_prefix;       // per walk: prefix value (or the whole value if the walk ended first)
   float* nc_xm;           // per walk: X_m (3 floats)
   uint8_t* nc_need;       // per walk: 1 if the cache must be added at X_m
@@ -80,6 +81,7 @@ struct WosRun {
   void* ev_end;
   int mode;
   int depth;
+  int antithetic;
   float* nc_prefix;
   float* nc_xm;
   uint8_t* nc_need;

@@ -25,6 +25,7 @@ from .wos import Boundary, PointStats, WalkParams
 
 NC_PREFIX = 1
 NC_COUPLED = 2
+NC_ANTITHETIC = 0x100  # OR-ed into the mode: mirrored first steps in walk pairs (2j, 2j+1)
 
 
 @dataclass
@@ -54,15 +55,17 @@ class NcSolver:
         return self._ws
 
     def walks(self, points, wpp: int, mode: int, depth: int, stream_id: int, seed: int | None = None,
-              point_offset: int = 0) -> WalkBatch:
+              point_offset: int = 0, antithetic: bool = False) -> WalkBatch:
         import torch
+        if antithetic:
+            mode |= NC_ANTITHETIC
         n = points.shape[0]
         dev = points.device
         total = n * wpp
         prefix = torch.empty(total, dtype=torch.float32, device=dev)
         xm = torch.empty((total, 3), dtype=torch.float32, device=dev)
         need = torch.empty(total, dtype=torch.uint8, device=dev)
-        full = torch.empty(total, dtype=torch.float32, device=dev) if mode == NC_COUPLED else None
+        full = torch.empty(total, dtype=torch.float32, device=dev) if (mode & ~NC_ANTITHETIC) == NC_COUPLED else None
         steps = torch.empty(n, dtype=torch.int64, device=dev)
         wp = self.params.to_c()
         wp.rng_stream = stream_id
@@ -88,18 +91,22 @@ class NcSolver:
         return stats
 
     # ------------------------------------------------------------ estimators
-    def nc(self, cache, points, wpp: int, depth: int, stream_id: int = 1, seed: int | None = None):
-        """Neural-cache (biased) estimate: per-point stats of S_m + C(X_m)."""
-        b = self.walks(points, wpp, NC_PREFIX, depth, stream_id, seed)
+    def nc(self, cache, points, wpp: int, depth: int, stream_id: int = 1, seed: int | None = None,
+           antithetic: bool = False):
+        """Neural-cache (biased) estimate: per-point stats of S_m + C(X_m).
+        With antithetic=True the walks come in pairs with mirrored first steps
+        (same expectation; the pair mean cancels the odd part of C around x)."""
+        b = self.walks(points, wpp, NC_PREFIX, depth, stream_id, seed, antithetic=antithetic)
         cv = cache.infer(b.xm)
         st = self.combine(b, cv, points.shape[0], wpp, NC_PREFIX)
         st.steps = b.steps
         return st, b
 
     def two_level(self, cache, points, n_coarse: int, n_pairs: int, depth: int, seed: int | None = None,
-                  coarse_stream: int = 1, pair_stream: int = 2):
-        """NC + 2LMC: returns (estimate per point, coarse stats, residual stats, pair batch)."""
-        coarse, _ = self.nc(cache, points, n_coarse, depth, coarse_stream, seed)
+                  coarse_stream: int = 1, pair_stream: int = 2, antithetic: bool = False):
+        """NC + 2LMC: returns (estimate per point, coarse stats, residual stats, pair batch).
+        antithetic applies to the coarse level's walks."""
+        coarse, _ = self.nc(cache, points, n_coarse, depth, coarse_stream, seed, antithetic=antithetic)
         pb = self.walks(points, n_pairs, NC_COUPLED, depth, pair_stream, seed)
         cv = cache.infer(pb.xm)
         resid = self.combine(pb, cv, points.shape[0], n_pairs, NC_COUPLED)

@@ -78,7 +78,7 @@ def lib():
         L.orc_prune_threshold.restype = C.c_float
         L.orc_walks.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_int, _f32p, C.c_int, C.c_uint64,
                                 C.c_uint32, _f32p, C.c_void_p, C.c_long, C.c_int, C.c_int, _f32p, _i32p,
-                                C.c_void_p, C.c_void_p, C.c_int]
+                                C.c_void_p, C.c_void_p, C.c_int, C.c_int]
         L.orc_welford.argtypes = [_f32p, C.c_long, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_chan_merge.argtypes = [C.c_double] * 6 + [_f64p]
         L.orc_point_stats.argtypes = [_f32p, C.c_long, C.c_int, _f64p, _f64p, _f64p]
@@ -207,7 +207,7 @@ class Bvh2:
 
     def walks(self, pts, wpp, *, max_steps=256, eps=1e-4, step_scale=0.999, bmode=BC_CONSTANT_ONE,
               bparams=None, poisson=False, seed=0, stream=0, point_ids=None, depth=-1, nthreads=1,
-              want_xm=False):
+              want_xm=False, antithetic=False):
         pts = _f32(pts).reshape(-1, 3)
         npts = pts.shape[0]
         prm = _f32(bparams if bparams is not None else np.zeros(1))
@@ -221,7 +221,8 @@ class Bvh2:
             pid = np.ascontiguousarray(point_ids, np.uint32)
         rc = lib().orc_walks(self.h, max_steps, eps, step_scale, bmode, prm, int(poisson), seed, stream, pts,
                              pid.ctypes.data if pid is not None else None, npts, wpp, depth, values, steps,
-                             status.ctypes.data, xm.ctypes.data if xm is not None else None, nthreads)
+                             status.ctypes.data, xm.ctypes.data if xm is not None else None, nthreads,
+                             int(antithetic))
         if rc != 0:
             raise ValueError("orc_walks failed")
         return values, steps, status, xm

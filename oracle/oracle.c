@@ -736,6 +736,7 @@ typedef struct {
   int poisson;
   uint32_t seed_lo, seed_hi;
   uint32_t stream;
+  int antithetic; /* walks 2j+1 take the negated step-0 draws of walk 2j */
 } orc_walk_opts;
 
 static inline float median3(float a, float b, float c) {
@@ -782,17 +783,23 @@ static int walk_one(const orc_bvh2* b, const orc_walk_opts* o, v3 p, uint32_t po
       return 2;
     }
     const uint32_t key[2] = {o->seed_lo, o->seed_hi};
-    const uint32_t ctr[4] = {(uint32_t)step, walk, point, 2u * o->stream};
+    /* antithetic pair: the odd walk mirrors its even partner's first step */
+    const int anti = o->antithetic && step == 0 && (walk & 1u);
+    const uint32_t cw = anti ? walk - 1u : walk;
+    const float sgn = anti ? -1.0f : 1.0f;
+    const uint32_t ctr[4] = {(uint32_t)step, cw, point, 2u * o->stream};
     uint32_t r[4];
     orc_philox4x32_10(ctr, key, r);
-    const v3 dir = dir_from(r[0], r[1]);
+    v3 dir = dir_from(r[0], r[1]);
+    dir = mk(sgn * dir.x, sgn * dir.y, sgn * dir.z);
     const float R = o->step_scale * d;
     if (o->poisson) {
-      const uint32_t ctr2[4] = {(uint32_t)step, walk, point, 2u * o->stream + 1u};
+      const uint32_t ctr2[4] = {(uint32_t)step, cw, point, 2u * o->stream + 1u};
       uint32_t t[4];
       orc_philox4x32_10(ctr2, key, t);
       const float rho = median3(u01(r[2]), u01(r[3]), u01(t[0]));
-      const v3 e = dir_from(t[1], t[2]);
+      v3 e = dir_from(t[1], t[2]);
+      e = mk(sgn * e.x, sgn * e.y, sgn * e.z);
       const float Rr = R * rho;
       const v3 y = mk(p.x + Rr * e.x, p.y + Rr * e.y, p.z + Rr * e.z);
       src = src + (R * R) * poisson_source_f32(y) / 6.0f;
@@ -862,12 +869,14 @@ static void* wk_worker2(void* arg) {
 }
 
 /* Runs npts * wpp walks; sample s = point * wpp + walk.  point_ids (optional)
- * gives the RNG point index of each point (default: its position). */
+ * gives the RNG point index of each point (default: its position).  With
+ * antithetic != 0, walk 2j+1 takes the negated step-0 draws of walk 2j. */
 int orc_walks(const orc_bvh2* b, int max_steps, float eps, float step_scale, int bmode, const float* bparams,
               int poisson, uint64_t seed, uint32_t stream, const float* pts, const uint32_t* point_ids, long npts,
-              int wpp, int depth, float* values, int* steps, int* status, float* xm, int nthreads) {
+              int wpp, int depth, float* values, int* steps, int* status, float* xm, int nthreads, int antithetic) {
   if (!b || wpp < 1 || npts < 0) return -1;
-  orc_walk_opts o = {max_steps, eps, step_scale, bmode, bparams, poisson, (uint32_t)seed, (uint32_t)(seed >> 32), stream};
+  orc_walk_opts o = {max_steps, eps, step_scale, bmode, bparams, poisson, (uint32_t)seed, (uint32_t)(seed >> 32), stream,
+                     antithetic};
   if (nthreads < 1) nthreads = 1;
   if (nthreads > 256) nthreads = 256;
   wk_shared sh;

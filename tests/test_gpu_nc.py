@@ -21,8 +21,8 @@ def m1(eng, orc, torch_gpu):
     return v, t, bvh, orc.Bvh2(v, t)
 
 
-@pytest.mark.parametrize("poisson", [False, True])
-def test_prefix_and_coupled_walks_bit_exact(eng, orc, torch_gpu, m1, poisson):
+@pytest.mark.parametrize("poisson,anti", [(False, False), (True, False), (False, True), (True, True)])
+def test_prefix_and_coupled_walks_bit_exact(eng, orc, torch_gpu, m1, poisson, anti):
     from n2wos_amd import nc as N, problems as P, wos as W
     torch = torch_gpu
     v, t, bvh, b2 = m1
@@ -31,12 +31,12 @@ def test_prefix_and_coupled_walks_bit_exact(eng, orc, torch_gpu, m1, poisson):
     wp = W.WalkParams(epsilon=1e-3, seed=21, source=W.SOURCE_POISSON_SINES if poisson else 0)
     s = N.NcSolver(bvh, bc, wp)
     tp = torch.from_numpy(pts).cuda()
-    print("case: nc walks poisson=%s" % poisson, flush=True)
+    print("case: nc walks poisson=%s antithetic=%s" % (poisson, anti), flush=True)
     for mode, stream in ((N.NC_PREFIX, 1), (N.NC_COUPLED, 2)):
-        b = s.walks(tp, 37, mode, 3, stream)
+        b = s.walks(tp, 37, mode, 3, stream, antithetic=anti)
         torch.cuda.synchronize()
         kw = dict(max_steps=256, eps=1e-3, step_scale=0.999, bmode=bc.mode, bparams=bc.oracle_params(),
-                  poisson=poisson, seed=21, stream=stream, nthreads=16)
+                  poisson=poisson, seed=21, stream=stream, nthreads=16, antithetic=anti)
         ov, os_, ost, oxm = b2.walks(pts, 37, depth=3, want_xm=True, **kw)
         need = b.need.cpu().numpy()
         assert np.array_equal(need, (ost == 2).astype(np.uint8))
@@ -47,6 +47,37 @@ def test_prefix_and_coupled_walks_bit_exact(eng, orc, torch_gpu, m1, poisson):
             fv, fs, _, _ = b2.walks(pts, 37, **kw)
             assert np.array_equal(bits(b.full.cpu().numpy()), bits(fv))
             assert np.array_equal(b.steps.cpu().numpy(), fs.reshape(64, 37).sum(1))
+
+
+def test_antithetic_pairs_unbiased_and_lower_variance(eng, torch_gpu, m1):
+    """Depth-1 NC with the exact solution as the cache: mirrored first steps
+    keep the estimate unbiased (z-scores of pair means against the FP64
+    solution) and cut the variance of a pair mean well below that of two
+    independent samples (the odd part of u on the first sphere cancels)."""
+    from n2wos_amd import nc as N, problems as P, wos as W
+    torch = torch_gpu
+    v, t, bvh, _ = m1
+    pts = P.interior_points(bvh, v, 1024, seed=13)
+    bc = P.charges_boundary(0)
+    s = N.NcSolver(bvh, bc, W.WalkParams(epsilon=1e-4, seed=17))
+    tp = torch.from_numpy(pts).cuda()
+    truth = P.charges_solution_f64(bc.charges, pts)
+    out = {}
+    for anti in (False, True):
+        b = s.walks(tp, 128, N.NC_PREFIX, 1, 1, antithetic=anti)
+        need = b.need.cpu().numpy().astype(bool)
+        xm = b.xm.cpu().numpy()
+        val = b.prefix.cpu().numpy().astype(np.float64)
+        val[need] += P.charges_solution_f64(bc.charges, xm[need])
+        out[anti] = val.reshape(1024, 64, 2).mean(2)  # pair means
+    for anti, pm in out.items():
+        se = pm.std(1, ddof=1) / np.sqrt(64)
+        z = (pm.mean(1) - truth) / np.maximum(se, 1e-12)
+        assert abs(z.mean()) < 5 / np.sqrt(len(z)), anti
+        assert np.mean(np.abs(z) > 3.5) < 0.02, anti
+    v_ind = out[False].var(1, ddof=1).mean()
+    v_anti = out[True].var(1, ddof=1).mean()
+    assert v_anti < 0.3 * v_ind, (v_anti, v_ind)
 
 
 def test_combine_matches_numpy(eng, torch_gpu, m1):
