@@ -281,6 +281,18 @@ typedef struct {
 } n2wos_cache_config;
 int n2wos_cache_create(const n2wos_cache_config* config, n2wos_cache** out);
 int n2wos_cache_destroy(n2wos_cache* cache);
+/* A constant added to the network's output, in inference and in the
+ * training loss (default 0).  Setting it to the mean training target lets
+ * the bias-free MLP fit the variation only, which trains in far fewer steps.
+ * No counterpart in the reference (tiny-cuda-nn's MLP is also bias-free;
+ * the reference trains on raw targets). */
+int n2wos_cache_set_output_offset(n2wos_cache* cache, float offset);
+/* Queue a device-to-device copy of src's parameters (grid, FP32 and FP16
+ * weights, output offset) into dst, which must have the same grid layout, on
+ * a stream.  A frozen inference snapshot while src keeps training: the
+ * two-cache snapshot discipline of docs/progressive_snapshot_accounting_policy.md
+ * (the reference describes it; it has no such entry point). */
+int n2wos_cache_copy_params(n2wos_cache* dst, const n2wos_cache* src, void* stream);
 /* d_out[i] = C(d_points[i]) for n points (12 B each). */
 int n2wos_cache_infer(const n2wos_cache* cache, const float* d_points, int n, float* d_out, void* stream);
 /* One Adam step on the batch (n points, n targets).  *d_loss_out (device,

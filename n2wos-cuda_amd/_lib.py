@@ -82,6 +82,8 @@ _SIGNATURES = {
     "n2wos_cache_infer": ([vp, vp, i32, vp, vp], i32),
     "n2wos_cache_train_step": ([vp, vp, vp, i32, vp, vp], i32),
     "n2wos_cache_layout": ([vp, vp, vp, vp, vp, vp, vp, vp], i32),
+    "n2wos_cache_set_output_offset": ([vp, f32], i32),
+    "n2wos_cache_copy_params": ([vp, vp, vp], i32),
     "n2wos_cache_get_params": ([vp, vp, vp], i32),
     "n2wos_cache_set_params": ([vp, vp, vp], i32),
     "n2wos_cache_gradients": ([vp, vp, vp, i32, vp, vp, vp], i32),

@@ -86,6 +86,20 @@ class Cache:
         return {"n_levels": nl.value, "table_size": t.value, "offsets": list(off), "sizes": list(sz),
                 "res": list(res), "dense": list(dense), "total_entries": tot.value}
 
+    def set_output_offset(self, offset: float) -> None:
+        """Constant added to the network output in inference and training
+        (normalise the targets by their mean before training)."""
+        check(lib().n2wos_cache_set_output_offset(self.handle, float(offset)))
+        self.output_offset = float(offset)
+
+    def copy_from(self, other: "Cache", stream=None) -> None:
+        """Queue a device copy of other's parameters into this cache (same
+        layout) on a stream: a frozen snapshot while other keeps training."""
+        import torch
+        s = torch.cuda.current_stream().cuda_stream if stream is None else getattr(stream, "cuda_stream", stream)
+        check(lib().n2wos_cache_copy_params(self.handle, other.handle, s))
+        self.output_offset = getattr(other, "output_offset", 0.0)
+
     def infer(self, points, out=None, stream=None):
         import torch
         n = points.shape[0]

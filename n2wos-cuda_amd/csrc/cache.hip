@@ -177,11 +177,11 @@ __device__ __forceinline__ void stage_weights(_Float16* lds, const _Float16* __r
   __syncthreads();
 }
 
-// Inference: out[s] = C(points[s]) for s < n.
+// Inference: out[s] = C(points[s]) (+ the output offset, when set) for s < n.
 __global__ void __launch_bounds__(kCacheBlock) k_cache_infer(GridLayout gl, const float* __restrict__ grid,
                                                             const _Float16* __restrict__ wh,
                                                             const float* __restrict__ points, int n,
-                                                            float* __restrict__ out) {
+                                                            float offset, float* __restrict__ out) {
   __shared__ _Float16 w[kCacheWeights];
   stage_weights(w, wh);
   const int lane = static_cast<int>(threadIdx.x & 63);
@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(kCacheBlock) k_cache_infer(GridLayout gl, cons
     encode_fragments(gl, grid, ux, uy, uz, g, xb);
     h4_t h1[4], h2[4], h3[4], h4[4];
     const f4_t y = mlp_forward(w, xb, i, g, h1, h2, h3, h4);
-    if (g == 0 && s < n) out[s] = y[0];
+    if (g == 0 && s < n) out[s] = offset != 0.0f ? y[0] + offset : y[0];
   }
 }
 
@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(kCacheBlock) k_cache_train_pass1(GridLayout gl
                                                                   const _Float16* __restrict__ wh,
                                                                   const float* __restrict__ points,
                                                                   const float* __restrict__ targets, int n,
-                                                                  float grad_scale, TrainStore st,
+                                                                  float offset, float grad_scale, TrainStore st,
                                                                   float* __restrict__ grid_grad,
                                                                   float* __restrict__ loss_sum) {
   __shared__ _Float16 w[kCacheWeights];
@@ -262,7 +262,7 @@ __global__ void __launch_bounds__(kCacheBlock) k_cache_train_pass1(GridLayout gl
     // dL/dy for L = mean (y - t)^2, scaled; broadcast from lane (i, 0) to (i, g)
     float dy = 0.0f;
     if (g == 0 && live) {
-      const float diff = y[0] - targets[s];
+      const float diff = (offset != 0.0f ? y[0] + offset : y[0]) - targets[s];
       loss += diff * diff;
       dy = 2.0f * diff * grad_scale;
     }
@@ -469,6 +469,7 @@ struct n2wos_cache {
   int store_batch = 0;
   int64_t step = 0;
   float lr = 1e-2f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-15f, loss_scale = 128.0f;
+  float out_offset = 0.0f;  // added to the network's output (inference and loss)
 };
 
 using namespace n2;
@@ -584,7 +585,7 @@ extern "C" int n2wos_cache_infer(const n2wos_cache* c, const float* d_points, in
     const int tiles = (n + 15) / 16;
     const int blocks = std::max(1, std::min(4096, (tiles + 3) / 4));
     hipLaunchKernelGGL(k_cache_infer, dim3(blocks), dim3(kCacheBlock), 0, static_cast<hipStream_t>(stream),
-                       c->layout, c->grid, c->wh, d_points, n, d_out);
+                       c->layout, c->grid, c->wh, d_points, n, c->out_offset, d_out);
     N2_HIP(hipGetLastError());
     return N2WOS_OK;
   });
@@ -618,7 +619,7 @@ extern "C" int n2wos_cache_train_step(n2wos_cache* c, const float* d_points, con
     const int tiles = batch / 16;
     const int blocks = std::max(1, std::min(2048, (tiles + 3) / 4));
     hipLaunchKernelGGL(k_cache_train_pass1, dim3(blocks), dim3(kCacheBlock), 0, s, c->layout, c->grid, c->wh,
-                       d_points, d_targets, n, grad_scale, st, c->grid_grad, c->loss);
+                       d_points, d_targets, n, c->out_offset, grad_scale, st, c->grid_grad, c->loss);
     const int chunk = 1024;
     const int waves = (batch + chunk - 1) / chunk;
     hipLaunchKernelGGL(k_cache_wgrad, dim3((waves + 3) / 4, 5), dim3(kCacheBlock), 0, s, st, chunk, c->w_grad);
@@ -638,6 +639,36 @@ extern "C" int n2wos_cache_train_step(n2wos_cache* c, const float* d_points, con
     hipLaunchKernelGGL(k_to_half, dim3((kCacheWeights + 255) / 256), dim3(256), 0, s, c->w, c->wh, kCacheWeights);
     N2_HIP(hipGetLastError());
     if (d_loss_out) N2_HIP(hipMemcpyAsync(d_loss_out, c->loss, 4, hipMemcpyDeviceToDevice, s));
+    return N2WOS_OK;
+  });
+}
+
+extern "C" int n2wos_cache_set_output_offset(n2wos_cache* c, float offset) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(c != nullptr, N2WOS_ERR_NULL, "NULL cache");
+    N2_REQUIRE(std::isfinite(offset), N2WOS_ERR_INVALID, "offset must be finite");
+    c->out_offset = offset;
+    return N2WOS_OK;
+  });
+}
+
+// Device-to-device copy of one cache's parameters (grid, FP32 and FP16
+// weights, output offset) into another of the same layout, queued on a
+// stream: a snapshot for inference while the source keeps training.
+extern "C" int n2wos_cache_copy_params(n2wos_cache* dst, const n2wos_cache* src, void* stream) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(dst && src, N2WOS_ERR_NULL, "NULL cache");
+    const GridLayout& a = dst->layout;
+    const GridLayout& b = src->layout;
+    bool same = a.n_levels == b.n_levels && a.table_size == b.table_size && a.total_entries == b.total_entries;
+    for (int l = 0; same && l < kCacheLevels; ++l) same = a.offset[l] == b.offset[l] && a.res[l] == b.res[l];
+    N2_REQUIRE(same, N2WOS_ERR_INVALID, "caches have different layouts");
+    if (dst == src) return N2WOS_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    N2_HIP(hipMemcpyAsync(dst->grid, src->grid, grid_floats(src) * 4, hipMemcpyDeviceToDevice, s));
+    N2_HIP(hipMemcpyAsync(dst->w, src->w, kCacheWeights * 4, hipMemcpyDeviceToDevice, s));
+    N2_HIP(hipMemcpyAsync(dst->wh, src->wh, kCacheWeights * 2, hipMemcpyDeviceToDevice, s));
+    dst->out_offset = src->out_offset;
     return N2WOS_OK;
   });
 }
@@ -712,7 +743,7 @@ extern "C" int n2wos_cache_gradients(n2wos_cache* c, const float* d_points, cons
     N2_HIP(hipMemset(c->loss, 0, 4));
     const int tiles = batch / 16;
     hipLaunchKernelGGL(k_cache_train_pass1, dim3(std::max(1, std::min(2048, (tiles + 3) / 4))), dim3(kCacheBlock), 0,
-                       nullptr, c->layout, c->grid, c->wh, d_points, d_targets, n,
+                       nullptr, c->layout, c->grid, c->wh, d_points, d_targets, n, c->out_offset,
                        c->loss_scale / static_cast<float>(n), st, c->grid_grad, c->loss);
     const int waves = (batch + 1023) / 1024;
     hipLaunchKernelGGL(k_cache_wgrad, dim3((waves + 3) / 4, 5), dim3(kCacheBlock), 0, nullptr, st, 1024, c->w_grad);

@@ -29,24 +29,33 @@ class CacheSetup:
     name: str
     depth: int = 4
     cache_kw: dict = field(default_factory=dict)  # CacheConfig overrides (n_levels, ...)
-    train_pairs: int = 32                          # coupled label walks per point
+    train_pairs: int = 32                          # coupled label walks per labelled point
+    label_stride: int = 1                          # label every k-th evaluation point
     train_steps: int = 200
     batch: int = 1 << 18
+    normalize_targets: bool = False                # cache output offset = mean label
+    antithetic: bool = False                       # mirrored first steps (labels and NC samples)
     nc_wpp: tuple = (64, 128, 256)
     lmc: tuple = ((256, 16), (512, 16))            # (coarse samples, coupled pairs) per point
 
 
 # The learned-cache configuration of the benchmark problem (16 levels x 2
 # features, T = 2^19, N_min 16, growth 1.5; 4 x 64 ReLU MLP; depth 4; 32
-# label walks per point; 16 coupled pairs per point for 2LMC) and two
-# shallower setups found on the same problem (depth 1; 8 or 4 levels, whose
-# coarser grids generalise better from single-walk labels; 100 Adam steps).
+# label walks per point; 16 coupled pairs per point for 2LMC), and shallow
+# setups found on the same problem: depth 1, coarse grids (2 or 3 levels
+# generalise best from single-walk labels), targets normalised by their mean,
+# 16 antithetic label pairs on every 4th point, 100 Adam steps at batch 2^15,
+# and NC samples in antithetic pairs (the pair mean cancels the odd part of
+# the cache around each point, so few samples reach the cache's bias floor).
 DEFAULT_SETUPS = (
     CacheSetup("depth4_levels16", depth=4, train_steps=200, nc_wpp=(64, 256), lmc=((256, 16), (512, 16))),
-    CacheSetup("depth1_levels8", depth=1, cache_kw={"n_levels": 8}, train_steps=100, nc_wpp=(128, 192, 256),
-               lmc=()),
-    CacheSetup("depth1_levels4", depth=1, cache_kw={"n_levels": 4}, train_steps=100, nc_wpp=(96, 128, 192, 256),
-               lmc=((128, 16),)),
+    CacheSetup("depth1_levels2_anti", depth=1, cache_kw={"n_levels": 2}, train_pairs=16, label_stride=4,
+               train_steps=100, batch=1 << 15, normalize_targets=True, antithetic=True, nc_wpp=(32, 64),
+               lmc=((64, 64),)),
+    CacheSetup("depth1_levels3_anti", depth=1, cache_kw={"n_levels": 3}, train_pairs=16, label_stride=4,
+               train_steps=100, batch=1 << 15, normalize_targets=True, antithetic=True, nc_wpp=(32, 64), lmc=()),
+    CacheSetup("depth1_levels4", depth=1, cache_kw={"n_levels": 4}, train_steps=100, batch=1 << 16,
+               nc_wpp=(96, 128, 192, 256), lmc=()),
 )
 
 
@@ -84,13 +93,19 @@ def run(bvh, verts, points, boundary, truth, params, tau=1e-2, plain_wpp=(256, 3
     out["setups"] = []
     for su in setups:
         res = {"name": su.name, "depth": su.depth, "cache": dict(su.cache_kw), "train_pairs": su.train_pairs,
-               "train_steps": su.train_steps, "batch": su.batch}
+               "label_stride": su.label_stride, "train_steps": su.train_steps, "batch": su.batch,
+               "normalize_targets": su.normalize_targets, "antithetic": su.antithetic}
         cfg = K.CacheConfig.for_bounds(verts.min(0), verts.max(0), seed=seed, **su.cache_kw)
         cache = K.Cache(cfg)
 
         def train():
-            b = ns.walks(tp, su.train_pairs, N.NC_COUPLED, su.depth, stream_id=7, seed=seed)
+            lp = tp[:: su.label_stride].contiguous() if su.label_stride > 1 else tp
+            b = ns.walks(lp, su.train_pairs, N.NC_COUPLED, su.depth, stream_id=7, seed=seed,
+                         antithetic=su.antithetic)
             x, y = ns.training_pairs(b)
+            if su.normalize_targets:
+                off = float(y.mean().item())
+                cache.set_output_offset(off)
             g = torch.Generator(device="cuda").manual_seed(seed)
             n = x.shape[0]
             for _ in range(su.train_steps):
@@ -116,14 +131,16 @@ def run(bvh, verts, points, boundary, truth, params, tau=1e-2, plain_wpp=(256, 3
 
         ncr = []
         for wpp in su.nc_wpp:
-            (st, _), dt = timed(lambda: ns.nc(cache, tp, wpp, su.depth, stream_id=1, seed=seed + wpp))
+            (st, _), dt = timed(lambda: ns.nc(cache, tp, wpp, su.depth, stream_id=1, seed=seed + wpp,
+                                              antithetic=su.antithetic))
             r = _rmse(st.mean.cpu().numpy(), truth)
             ncr.append({"wpp": wpp, "seconds": dt, "seconds_with_training": dt + t_train, "rmse": r})
             log(f"[{su.name}] nc wpp={wpp} t={dt:.3f}s (+train {t_train:.3f}s) rmse={r:.4g}")
         res["nc"] = ncr
         tl = []
         for nc_, nr in su.lmc:
-            (r2, dt) = timed(lambda: ns.two_level(cache, tp, nc_, nr, su.depth, seed=seed + nc_ + 1000 * nr))
+            (r2, dt) = timed(lambda: ns.two_level(cache, tp, nc_, nr, su.depth, seed=seed + nc_ + 1000 * nr,
+                                                  antithetic=su.antithetic))
             est, co, re, _ = r2
             r = _rmse(est.cpu().numpy(), truth)
             vc = float((co.m2 / (co.n - 1)).mean().item())

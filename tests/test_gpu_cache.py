@@ -118,3 +118,34 @@ def test_cache_overfits_smooth_harmonic(eng, torch_gpu):
     rmse = torch.sqrt(torch.mean((y - (q[:, 0] ** 2 - q[:, 1] ** 2)) ** 2)).item()
     print("overfit rmse", rmse)
     assert rmse < 0.02
+
+
+def test_output_offset_in_inference_and_loss(eng, torch_gpu):
+    """The output offset adds exactly to inference, a zero offset changes
+    nothing bit for bit, and training with offset o on targets t follows
+    training without it on t - o (the loss sees (y + o) - t): after three Adam
+    steps the two networks' learned changes agree to 1e-2 relative."""
+    from n2wos_amd import cache as K
+    torch = torch_gpu
+    cfg = K.CacheConfig(n_levels=4, log2_table_size=12, base_resolution=4, growth=1.5, seed=9)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    x = (torch.rand((4096, 3), device="cuda", generator=g) * 2.2 - 1.1).contiguous()
+    t = (torch.sin(2 * x[:, 0]) * x[:, 1] + 3.0).contiguous()
+    a, b = K.Cache(cfg), K.Cache(cfg)
+    y0 = a.infer(x)
+    a.set_output_offset(3.0)
+    y1 = a.infer(x)
+    assert torch.equal(y1, y0 + 3.0)
+    a.set_output_offset(0.0)
+    assert torch.equal(a.infer(x), y0)
+    a.set_output_offset(3.0)
+    t_shift = (t - 3.0).contiguous()
+    for _ in range(3):
+        a.train_step(x, t)
+        b.train_step(x, t_shift)
+    da = (a.infer(x) - 3.0 - y0).cpu().numpy()
+    db = (b.infer(x) - y0).cpu().numpy()
+    assert np.linalg.norm(db) > 1e-3
+    assert rel(da, db) < 1e-2
+    with pytest.raises(Exception):
+        a.set_output_offset(float("nan"))

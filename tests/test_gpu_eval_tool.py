@@ -1,0 +1,61 @@
+"""The evaluation tool end to end on a small problem: the v4 JSON keys the
+reference's analysis scripts read, the estimates CSV columns, the slice mask
+and points files, and sane numbers (NC and 2LMC near the FP64 solution)."""
+import csv
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TOOL = os.path.join(ROOT, "n2wos-cuda_amd", "tools", "n2wos_eval_nc_wos.py")
+
+
+def test_eval_tool_slice_outputs(tmp_path, torch_gpu):
+    out = tmp_path / "eval.json"
+    pre = str(tmp_path / "est")
+    args = [sys.executable, TOOL, "--mesh", "mesh1", "--bc", "harmonic_x2_minus_y2", "--eval-mode", "slice",
+            "--slice-width", "48", "--slice-height", "40", "--train-points", "4096", "--label-refreshes", "2",
+            "--walks-per-label-refresh", "16", "--train-steps-per-refresh", "60", "--n-levels", "4",
+            "--pure-walks-per-point", "64", "--hybrid-walks-per-point", "32", "--coarse-walks-per-point", "32",
+            "--residual-walks-per-point", "16", "--depth-m", "1", "--epsilon", "1e-3", "--output", str(out),
+            "--save-estimates-prefix", pre, "--save-train-points-prefix", pre]
+    print("case: eval tool slice 48x40 mesh1", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    doc = json.loads(out.read_text())
+    assert doc["schema"] == "n2wos_tcnn_nc_wos_eval_v4"
+    for k in ("implementation_mode", "options", "slice_eval", "training_point_sampler", "mesh_stats", "bvh_stats",
+              "training", "runs", "comparison", "estimate_outputs"):
+        assert k in doc, k
+    assert doc["implementation_mode"]["cache_backend"] == "hip_hashgrid_mlp_mfma_gfx950"
+    runs = doc["runs"]
+    for k in ("pure_wos", "nc_wos", "nc_2lmc", "nc_2lmc_m1"):
+        assert k in runs, k
+    for k in ("rmse", "mae", "max_abs_error", "mean_bias", "mean_sample_variance", "mean_steps", "elapsed_ms",
+              "training_plus_elapsed_ms", "us_per_sample", "cache_query_fraction"):
+        assert k in runs["pure_wos"] and k in runs["nc_wos"], k
+    n = doc["slice_eval"]["inside_pixels"]
+    assert 0 < n < 48 * 40 and runs["pure_wos"]["eval_points"] == n
+    assert runs["pure_wos"]["rmse"] < 0.05 and runs["nc_2lmc"]["rmse"] < 0.1
+    assert runs["nc_wos"]["cache_query_fraction"] > 0.9
+    with open(pre + "_estimates.csv") as f:
+        rows = list(csv.reader(f))
+    assert rows[0] == ["point_id", "x", "y", "z", "analytic_value", "pure_mean", "pure_sample_variance",
+                       "nc_wos_mean", "nc_wos_sample_variance", "nc_2lmc_mean", "nc_2lmc_coarse_mean",
+                       "nc_2lmc_residual_mean", "nc_2lmc_coarse_sample_variance", "nc_2lmc_residual_sample_variance"]
+    assert len(rows) == n + 1
+    a = np.array(rows[1:], np.float64)
+    assert np.allclose(a[:, 4], a[:, 1] ** 2 - a[:, 2] ** 2, atol=1e-6)
+    assert np.allclose(a[:, 9], a[:, 10] + a[:, 11], atol=1e-6)
+    mask = (tmp_path / "eval_slice_mask.ppm").read_bytes()
+    assert mask.startswith(b"P6\n48 40\n255\n") and len(mask) == len(b"P6\n48 40\n255\n") + 48 * 40 * 3
+    with open(tmp_path / "eval_slice_points.csv") as f:
+        assert len(f.read().strip().splitlines()) == n + 1
+    with open(pre + "_train_points.csv") as f:
+        assert len(f.read().strip().splitlines()) == 4096 + 1
