@@ -19,6 +19,9 @@ duration measured by HIP events on its own stream.  `cpu_baseline` times the
 oracle's CPU walker (same walks, same Philox streams) on a sample of the
 points.  Extra figures sit under "extra":
   closest_point_problem1   65,536 uniform queries on the 64 x 128 sphere
+  closest_point_problem1_cpu  the reference's CPU brute force (compiled from
+                           its sources) and the oracle's brute force / binary
+                           BVH on 1 and on all host threads, bounded samples
   closest_point_stress     262,144 queries each from uniform box, near-
                            boundary shell (|d| < 0.02) and WoS prefixes (1-20
                            steps) on the 128 x 256 sphere: queries/s and HBM
@@ -32,11 +35,15 @@ points.  Extra figures sit under "extra":
   time_to_rmse             plain WoS vs neural cache (NC) and NC + 2LMC to
                            RMSE <= 1e-2 on the headline problem, cache
                            training included in the NC times
+  progressive_2lmc         NC + 2LMC in blocks while a second cache trains on
+                           another stream (snapshot overlap) vs the same
+                           schedule on one stream: wall times, RMSE, identity
   cache                    hash-grid + MLP training step and inference rates
   serving                  request server: three resident meshes, seeded
                            Poisson arrivals of log-uniform request sizes,
                            p50 / p99 latency and throughput
---quick skips the heavy sections (problem 3, time-to-RMSE, serving).
+--quick skips the heavy sections (problem 3, time-to-RMSE, progressive,
+serving); --section runs one section's GPU work alone (counter profiles).
 """
 from __future__ import annotations
 
@@ -72,6 +79,10 @@ def parse():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run a 1 GiB device copy (counter calibration) and one walker solve, then exit; "
                          "used under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--section", choices=["walker", "cache", "closest_stress"], default=None,
+                    help="run one section's GPU work only (one headline solve; the cache's training steps and "
+                         "inference; the closest-point stress sets), print a short JSON line and exit: the "
+                         "workload of the counter profiles")
     return ap.parse_args()
 
 
@@ -96,6 +107,17 @@ def timed_events(torch, fn, reps: int) -> float:
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def achievable_hbm_gbs(torch, nbytes: int = 1 << 30, reps: int = 10) -> float:
+    """Achievable HBM bandwidth: a device-to-device copy of 1 GiB (read +
+    write bytes / time), the STREAM-copy denominator of the roofline."""
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    ms = timed_events(torch, lambda: y.copy_(x), reps)
+    del x, y
+    return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
 def closest_point_stress(eng, torch, P, n: int = 262144) -> dict:
@@ -134,8 +156,8 @@ def cpu_closest_point(P, threads: int, gpu_qps: float, sample: int = 2048) -> di
     (src/geometry/closest_point.cpp:50-68), single-threaded as in its probe
     (src/tools/probe_cuda_bvh.cpp:153-167) -- timed on a bounded sample of the
     problem-1 queries, beside the oracle's brute force and its binary-BVH
-    traversal on 1 and on all host threads.  Every CPU result is checked
-    equal to the GPU's on the sample."""
+    traversal on 1 and on all host threads.  The reference's and the BVH's
+    results are checked equal to the oracle brute force on the sample."""
     import oracle
     oracle.build()
     v1, t1 = P.make_mesh("mesh1")
@@ -281,6 +303,18 @@ def main():
     stats = [W.PointStats.zeros(n) for _ in range(args.steps)]
     events = [hiptime.EventPair() for _ in range(args.steps)]
 
+    if args.section == "walker":
+        solver.solve(tp, args.wpp, seed=params.seed)
+        torch.cuda.synchronize()
+        print(json.dumps({"section": "walker", "points": n, "walks_per_point": args.wpp}))
+        return
+    if args.section == "cache":
+        print(json.dumps({"section": "cache", **cache_rates(eng, torch)}))
+        return
+    if args.section == "closest_stress":
+        print(json.dumps({"section": "closest_stress", **closest_point_stress(eng, torch, P)}))
+        return
+
     if args.traffic_probe:
         x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
         y = torch.empty_like(x)
@@ -327,6 +361,7 @@ def main():
     truth = P.charges_solution_f64(bc.charges, pts)
     mean = stats[0].mean.cpu().numpy()
     rmse = float(np.sqrt(np.mean((mean - truth) ** 2)))
+    achievable = achievable_hbm_gbs(torch)
 
     # closest-point queries/s: benchmark problem 1 (65,536 uniform queries)
     v1, t1 = P.make_mesh("mesh1")
@@ -372,6 +407,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
+            "achievable_GBps": achievable,
+            "frac_of_achievable": achieved / achievable,
             "traffic": traffic,
             "kernel": "k_wos_persistent",
             "kernel_ms_avg": avg_ms,
@@ -400,6 +437,8 @@ def main():
         ttr_params = W.WalkParams(max_steps=256, epsilon=1e-4, step_scale=0.999, seed=2024)
         r = ttrmse.run(bvh, verts, pts, bc, truth, ttr_params, plain_wpp=(320, 384, 448, 512), log=log)
         extra["time_to_rmse"] = r
+        from n2wos_amd import progressive as PR
+        extra["progressive_2lmc"] = PR.progressive_vs_serial(bvh, verts, pts, bc, truth, ttr_params, log=log)
         from n2wos_amd import serve as S
         extra["serving"] = S.serving_workload(eng, n_requests=args.serve_requests, rate_per_s=args.serve_rate,
                                               config=S.ServerConfig(hbm_budget_bytes=1 << 30, tick_walks=1 << 22),

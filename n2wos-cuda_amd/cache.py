@@ -58,6 +58,7 @@ class CacheConfig:
 class Cache:
     def __init__(self, config: CacheConfig | None = None):
         self.config = config or CacheConfig()
+        self.output_offset = 0.0
         h = C.c_void_p()
         self._cfg = self.config.to_c()
         check(lib().n2wos_cache_create(C.byref(self._cfg), C.byref(h)))

@@ -127,7 +127,7 @@ __device__ __forceinline__ Hit bvh4_closest(const Bvh4Node* __restrict__ nodes, 
           if (box_dist2(p, bxl, byl, bzl, bxh, byh, bzh) > thr) continue;
           const V3 a = v3(t0.x, t0.y, t0.z), b = v3(t0.w, t1.x, t1.y), c = v3(t1.z, t1.w, t2.x);
           const int id = __float_as_int(t2.y);
-          const V3 q = closest_on_triangle(p, a, b, c);
+          const V3 q = closest_on_triangle_flat(p, a, b, c);
           const V3 e = q - p;
           const float d2 = dot(e, e);
           if (d2 < h.d2 || (d2 == h.d2 && id < h.id)) {

@@ -4,6 +4,7 @@
 #pragma once
 
 #include "traverse.hpp"
+#include "wave.hpp"
 #include "wos.hpp"
 
 namespace n2 {
@@ -13,22 +14,6 @@ namespace n2 {
 #define N2_EMU_TRACE(sN, sT, sU)
 #endif
 
-#ifndef N2_EMULATED_WAVE
-__device__ __forceinline__ int lane_id() {
-  return static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)));
-}
-
-// number of set bits of `mask` below this lane
-__device__ __forceinline__ int rank_in(uint64_t mask) {
-  return static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
-                                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u)));
-}
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-#endif
 
 // Chan et al. pairwise merge of (n, mean, M2); same expression order as the oracle.
 __host__ __device__ __forceinline__ void chan_merge(double na, double ma, double sa, double nb, double mb, double sb,

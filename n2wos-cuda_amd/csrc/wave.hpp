@@ -1,0 +1,29 @@
+// wave.hpp -- wave-level helpers of the persistent kernels (64-lane waves).
+// The CPU SIMT emulator used by the tests defines N2_EMULATED_WAVE and
+// supplies its own versions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace n2 {
+
+#ifndef N2_EMULATED_WAVE
+__device__ __forceinline__ int lane_id() {
+  return static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)));
+}
+
+// number of set bits of `mask` below this lane
+__device__ __forceinline__ int rank_in(uint64_t mask) {
+  return static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u)));
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+#endif
+
+}  // namespace n2

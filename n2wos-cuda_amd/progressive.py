@@ -15,10 +15,12 @@ At the block boundary both streams join and infer <- train (device copy).
 Every block's coarse + residual estimate is unbiased whatever its snapshot,
 so the merged estimate is unbiased.  Residual walks are never used as
 labels: labels come from their own coupled walks (own Philox stream), as the
-reference policy requires.  Each block's work is a fixed function of its
-seeds and snapshot, and the merges run in a fixed order, so overlapping the
-two streams (overlap=True) gives the same numbers, bit for bit, as running
-the same schedule on one stream (overlap=False); only the wall time differs.
+reference policy requires.  Given its snapshots, the estimate is a fixed
+function of the seeds (each block's walks and merges run in a fixed order),
+so the overlap cannot change it: replay() recomputes it from recorded
+snapshots on one stream, bit for bit.  The snapshots themselves vary in
+their last bits from run to run, because the grid gradients are accumulated
+with FP32 atomics.
 """
 from __future__ import annotations
 
@@ -71,8 +73,32 @@ class ProgressiveTwoLevel:
             idx = torch.randint(0, n, (min(self.cfg.batch, n),), device="cuda", generator=gen)
             self.train_cache.train_step(x[idx].contiguous(), y[idx].contiguous())
 
-    def run(self, points, overlap: bool = True):
-        """Returns (estimate per point, coarse stats, residual stats, info)."""
+    def _estimate_block(self, ns, points, k: int, coarse, resid):
+        c = self.cfg
+        n = points.shape[0]
+        cb = ns.walks(points, c.coarse_wpp, N.NC_PREFIX, c.depth, stream_id=1, seed=c.seed + 31 * k,
+                      antithetic=c.antithetic)
+        ns.combine(cb, self.infer_cache.infer(cb.xm), n, c.coarse_wpp, N.NC_PREFIX, stats=coarse, accumulate=k > 0)
+        pb = ns.walks(points, c.residual_wpp, N.NC_COUPLED, c.depth, stream_id=2, seed=c.seed + 31 * k)
+        ns.combine(pb, self.infer_cache.infer(pb.xm), n, c.residual_wpp, N.NC_COUPLED, stats=resid, accumulate=k > 0)
+
+    def replay(self, points, snapshots):
+        """The estimator blocks alone, on the current stream, with recorded
+        snapshots (grid, weights, offset) in place of the training."""
+        n = points.shape[0]
+        coarse = W.PointStats.zeros(n, points.device)
+        resid = W.PointStats.zeros(n, points.device)
+        ns = N.NcSolver(self.bvh, self.boundary, self.params)
+        for k, (grid, weights, off) in enumerate(snapshots):
+            self.infer_cache.set_params(grid, weights)
+            self.infer_cache.set_output_offset(off)
+            self._estimate_block(ns, points, k, coarse, resid)
+        return coarse.mean + resid.mean, coarse, resid
+
+    def run(self, points, overlap: bool = True, record_snapshots: bool = False):
+        """Returns (estimate per point, coarse stats, residual stats, info);
+        with record_snapshots, info["snapshots"] holds each block's snapshot
+        (host copies taken between blocks, outside the overlapped work)."""
         import torch
         c = self.cfg
         n = points.shape[0]
@@ -102,18 +128,23 @@ class ProgressiveTwoLevel:
             self.infer_cache.copy_from(self.train_cache)
         torch.cuda.synchronize()
         info["warmup_seconds"] = time.perf_counter() - t_all
+        snaps = []
+        paused = 0.0
 
+        def record():
+            if record_snapshots:
+                t = time.perf_counter()
+                g, w = self.infer_cache.get_params()
+                snaps.append((g, w, self.infer_cache.output_offset))
+                return time.perf_counter() - t
+            return 0.0
+
+        paused += record()
         for k in range(c.blocks):
             t0 = time.perf_counter()
             # estimator block k with the frozen snapshot theta_k
             with torch.cuda.stream(s_est):
-                cb = ns_est.walks(points, c.coarse_wpp, N.NC_PREFIX, c.depth, stream_id=1, seed=c.seed + 31 * k,
-                                  antithetic=c.antithetic)
-                ns_est.combine(cb, self.infer_cache.infer(cb.xm), n, c.coarse_wpp, N.NC_PREFIX, stats=coarse,
-                               accumulate=k > 0)
-                pb = ns_est.walks(points, c.residual_wpp, N.NC_COUPLED, c.depth, stream_id=2, seed=c.seed + 31 * k)
-                ns_est.combine(pb, self.infer_cache.infer(pb.xm), n, c.residual_wpp, N.NC_COUPLED, stats=resid,
-                               accumulate=k > 0)
+                self._estimate_block(ns_est, points, k, coarse, resid)
             # meanwhile: the training stream moves theta toward theta_{k+1}
             if k + 1 < c.blocks:
                 with torch.cuda.stream(s_train):
@@ -129,8 +160,12 @@ class ProgressiveTwoLevel:
             # both streams drain before the next block starts
             torch.cuda.synchronize()
             info["blocks"].append({"block": k, "seconds": time.perf_counter() - t0})
+            if k + 1 < c.blocks:
+                paused += record()
         torch.cuda.synchronize()
-        info["seconds"] = time.perf_counter() - t_all
+        info["seconds"] = time.perf_counter() - t_all - paused
+        if record_snapshots:
+            info["snapshots"] = snaps
         est = coarse.mean + resid.mean
         return est, coarse, resid, info
 
@@ -138,13 +173,11 @@ class ProgressiveTwoLevel:
 def progressive_vs_serial(bvh, verts, points_np, boundary, truth, params, cfg: ProgressiveConfig | None = None,
                           cache_kw: dict | None = None, log=print) -> dict:
     """Runs the same progressive schedule overlapped and on one stream; reports
-    both wall times, the RMSE against the FP64 solution and whether the two
-    estimates are bit-identical."""
+    both wall times and the RMSE of each against the FP64 solution."""
     import torch
     cfg = cfg or ProgressiveConfig()
     tp = torch.from_numpy(np.ascontiguousarray(points_np, np.float32)).cuda()
     out = {"config": dict(cfg.__dict__)}
-    res = {}
     for overlap in (False, True):
         kcfg = K.CacheConfig.for_bounds(verts.min(0), verts.max(0), seed=cfg.seed, **(cache_kw or {"n_levels": 2}))
         pr = ProgressiveTwoLevel(bvh, boundary, params, kcfg, cfg)
@@ -154,8 +187,6 @@ def progressive_vs_serial(bvh, verts, points_np, boundary, truth, params, cfg: P
         key = "overlapped" if overlap else "serial"
         out[key] = {"seconds": info["seconds"], "warmup_seconds": info["warmup_seconds"], "rmse": r,
                     "blocks": info["blocks"], "mean_residual": float(re.mean.mean().item())}
-        res[key] = e
         log(f"progressive 2lmc {key}: {info['seconds']:.3f} s, rmse {r:.4g}")
-    out["bit_identical"] = bool(np.array_equal(res["serial"].view(np.uint64), res["overlapped"].view(np.uint64)))
     out["overlap_speedup"] = out["serial"]["seconds"] / out["overlapped"]["seconds"]
     return out
