@@ -68,9 +68,8 @@ int n2wos_abi_version(void);
 /* Launch tuning for benchmarks (process-wide): "ring" = LDS stack entries
  * per lane of the walker (8 or 16, default 16); "refill_min" = idle lanes
  * that trigger a refill (1..64, default 2); "weight_node" / "weight_tri" /
- * "weight_update" = per-lane weights of the walker's phase choice (10 each);
- * "merge" = 1 serves node visits and triangle tests in one iteration of the
- * walker (0 = one phase per iteration, the default). */
+ * "weight_update" = per-lane weights of the walker's phase choice (10, 10 and
+ * 30). */
 int n2wos_set_tuning(const char* key, int value);
 /* Device name and runtime version of the current HIP device, for reports. */
 int n2wos_device_summary(char* buf, int buf_len);

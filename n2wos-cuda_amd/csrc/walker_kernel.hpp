@@ -63,7 +63,7 @@ constexpr int kUpdate = 3;  // query finished: take the walk step (or end the wa
 // for paths they do not take, and a lane whose query ends starts its next
 // query without waiting for the rest of the wave.  Each lane's own sequence of
 // node visits, triangle tests and walk steps is exactly the sequential one.
-template <int CAP, int S, bool POISSON, bool COUNT, bool MERGE = false>
+template <int CAP, int S, bool POISSON, bool COUNT>
 __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persistent(WosKernelArgs a) {
   __shared__ uint32_t s_ref[CAP * kBlock];
   __shared__ float s_d2[CAP * kBlock];
@@ -293,8 +293,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
     const int sU = __popcll(mU) * a.weight_update;
     N2_EMU_TRACE(sN, sT, sU);
 
-    // the walk step is served when its lanes outweigh the traversal's
-    if (MERGE ? (sU >= sN + sT) : (sU >= sN && sU >= sT)) {
+    if (sU >= sN && sU >= sT) {
       // ------------------------------------------------------ walk step
       bool unit_done = false;
       if (phase == kUpdate) {
@@ -417,30 +416,6 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
           s_unit[wv][s] = -1;
         }
         wave_sync();
-      }
-    } else if (MERGE) {
-      // ------------------------------------- node visits and triangle tests
-      // together: node lanes and triangle lanes issue their loads at once,
-      // so both wait for one memory latency per iteration
-      const bool isN = phase == kNode, isT = phase == kTri;
-      float4 m0, m1, m2, m3, m4, m5, m6;
-      if (isN || isT) {
-        const float4* mp = isN ? reinterpret_cast<const float4*>(a.nodes + cur)
-                               : reinterpret_cast<const float4*>(a.tris + ti);
-        m0 = mp[0];
-        m1 = mp[1];
-        m2 = mp[2];
-        if (isN) {
-          m3 = mp[3];
-          m4 = mp[4];
-          m5 = mp[5];
-          m6 = mp[6];
-        }
-      }
-      if (isN) {
-        N2_NODE_VISIT(m0, m1, m2, m3, m4, m5, m6);
-      } else if (isT) {
-        N2_TRI_TEST(m0, m1, m2);
       }
     } else if (sN >= sT) {
       // ------------------------------------------------------ one node visit

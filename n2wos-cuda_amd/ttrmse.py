@@ -42,18 +42,19 @@ class CacheSetup:
 # The learned-cache configuration of the benchmark problem (16 levels x 2
 # features, T = 2^19, N_min 16, growth 1.5; 4 x 64 ReLU MLP; depth 4; 32
 # label walks per point; 16 coupled pairs per point for 2LMC), and shallow
-# setups found on the same problem: depth 1, coarse grids (2 or 3 levels
+# setups found on the same problem: depth 1, a 2-level grid (coarse grids
 # generalise best from single-walk labels), targets normalised by their mean,
-# 16 antithetic label pairs on every 4th point, 100 Adam steps at batch 2^15,
-# and NC samples in antithetic pairs (the pair mean cancels the odd part of
-# the cache around each point, so few samples reach the cache's bias floor).
+# 16 antithetic label pairs on every 8th (or 4th) point, 60 (100) Adam steps
+# at batch 2^15, and NC samples in antithetic pairs (the pair mean cancels the
+# odd part of the cache around each point, so few samples reach the cache's
+# bias floor).
 DEFAULT_SETUPS = (
     CacheSetup("depth4_levels16", depth=4, train_steps=200, nc_wpp=(64, 256), lmc=((256, 16), (512, 16))),
-    CacheSetup("depth1_levels2_anti", depth=1, cache_kw={"n_levels": 2}, train_pairs=16, label_stride=4,
-               train_steps=100, batch=1 << 15, normalize_targets=True, antithetic=True, nc_wpp=(32, 64),
-               lmc=((64, 64),)),
-    CacheSetup("depth1_levels3_anti", depth=1, cache_kw={"n_levels": 3}, train_pairs=16, label_stride=4,
-               train_steps=100, batch=1 << 15, normalize_targets=True, antithetic=True, nc_wpp=(32, 64), lmc=()),
+    CacheSetup("depth1_levels2_anti", depth=1, cache_kw={"n_levels": 2}, train_pairs=16, label_stride=8,
+               train_steps=60, batch=1 << 15, normalize_targets=True, antithetic=True, nc_wpp=(24, 32), lmc=()),
+    CacheSetup("depth1_levels2_anti_more_labels", depth=1, cache_kw={"n_levels": 2}, train_pairs=16,
+               label_stride=4, train_steps=100, batch=1 << 15, normalize_targets=True, antithetic=True,
+               nc_wpp=(32, 64), lmc=((64, 64),)),
     CacheSetup("depth1_levels4", depth=1, cache_kw={"n_levels": 4}, train_steps=100, batch=1 << 16,
                nc_wpp=(96, 128, 192, 256), lmc=()),
 )

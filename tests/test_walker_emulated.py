@@ -91,4 +91,4 @@ def test_emulated_walker_extreme_schedules(eng, orc, emu, weights, refill):
 
 def test_emulated_walker_sah_tree(eng, orc, emu):
     """Walks over a surface-area-heuristic tree (leaf 4) equal the oracle's."""
-    _case(eng, orc, emu, "mesh2", 5, 64, False, weights=(10, 10, 10), refill=2, eps=1e-4, build_method="sah", leaf=4)
+    _case(eng, orc, emu, "mesh2", 5, 64, False, weights=(10, 10, 30), refill=2, eps=1e-4, build_method="sah", leaf=4)
