@@ -66,7 +66,7 @@ typedef struct n2wos_bvh n2wos_bvh;
 const char* n2wos_last_error(void);
 int n2wos_abi_version(void);
 /* Launch tuning for benchmarks (process-wide): "ring" = LDS stack entries
- * per lane of the walker (8 or 16, default 16); "refill_min" = idle lanes
+ * per lane of the walker (8 or 16, default 8); "refill_min" = idle lanes
  * that trigger a refill (1..64, default 2); "weight_node" / "weight_tri" /
  * "weight_update" = per-lane weights of the walker's phase choice (10, 10 and
  * 30). */

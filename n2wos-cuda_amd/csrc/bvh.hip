@@ -23,7 +23,7 @@ __global__ void __launch_bounds__(1024) k_closest_point(const Bvh4Node* __restri
                                                         int* __restrict__ out_tri, int* __restrict__ out_overflow,
                                                         uint32_t* __restrict__ out_nodes,
                                                         uint32_t* __restrict__ out_tris) {
-  extern __shared__ uint32_t lds[];
+  extern __shared__ uint2 lds[];
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const V3 p = v3(points[3 * q], points[3 * q + 1], points[3 * q + 2]);
@@ -38,9 +38,7 @@ __global__ void __launch_bounds__(1024) k_closest_point(const Bvh4Node* __restri
   }
   Hit h;
   if (finite3(p)) {
-    uint32_t* sref = lds + threadIdx.x;
-    float* sd2 = reinterpret_cast<float*>(lds + kQueryCap * blockDim.x) + threadIdx.x;
-    h = bvh4_closest<kQueryCap, COUNT>(nodes, tris, p, delta, FLT_MAX, sref, sd2, blockDim.x);
+    h = bvh4_closest<kQueryCap, COUNT>(nodes, tris, p, delta, FLT_MAX, lds + threadIdx.x, blockDim.x);
   } else {
     h.d2 = FLT_MAX; h.cp = v3(0.0f, 0.0f, 0.0f); h.id = -1; h.overflow = 0; h.nodes = 0; h.tris = 0;
   }
@@ -125,7 +123,7 @@ int launch_query(const n2wos_bvh* bvh, const float* d_points, const uint8_t* d_a
   N2_REQUIRE(!MASKED || d_active, N2WOS_ERR_NULL, "query_device_masked received a null mask");
   N2_REQUIRE(!COUNT || (d_nodes && d_tris), N2WOS_ERR_NULL, "counted query received a null counter pointer");
   N2_REQUIRE(block >= 1 && block <= 1024, N2WOS_ERR_BLOCK, "block size must be in [1, 1024]");
-  const size_t lds = static_cast<size_t>(2 * kQueryCap) * static_cast<size_t>(block) * sizeof(uint32_t);
+  const size_t lds = static_cast<size_t>(kQueryCap) * static_cast<size_t>(block) * sizeof(uint2);
   auto kern = k_closest_point<MASKED, COUNT>;
   if (lds > 65536) {
     N2_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,

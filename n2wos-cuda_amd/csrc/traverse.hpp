@@ -1,17 +1,21 @@
 // traverse.hpp -- exact closest-point traversal of the 4-wide BVH on gfx950.
 //
 // One lane = one query.  The traversal stack holds (node ref, squared box
-// distance) pairs.  Its top CAP entries live in an LDS ring laid out
-// [slot][lane], so the 64 lanes of a wave always hit 64 different banks; when
-// a push meets a full ring, the ring's oldest entry moves to a per-lane spill
+// distance) pairs, one 8-byte entry each, so a push or pop is one LDS or
+// scratch access.  Its top CAP entries live in an LDS ring laid out
+// [slot][lane], so the lanes of a wave always hit consecutive banks; when a
+// push meets a full ring, the ring's oldest entry moves to a per-lane spill
 // array (scratch memory, touched only by deep traversals).  Pops take the
 // ring first, then the spill array, so the visit order is exactly that of one
-// unbounded stack of CAP + kSpill entries; 3 x (4-wide depth) <= that total is
-// checked when the BVH is built, so the overflow flag stays 0.  Children are
-// visited nearest first; a popped entry whose stored distance already exceeds
-// the pruning threshold is discarded without touching its node.  Ties in
-// squared distance go to the lowest original triangle index, which
-// reproduces the brute-force scan of the reference oracle
+// stack of CAP + kSpill entries.  A node visit leaves at most 3 entries
+// behind, so a tree of up to (CAP + kSpill) / 3 levels (18 for the walker's
+// ring of 8, 21 for the query kernel's 16) never overflows; the benchmark
+// meshes build 10 to 17 levels.  A deeper tree whose traversal fills the spill
+// array sets the overflow flag, like the reference's bounded stack.  Children
+// are visited nearest first; a popped entry whose stored distance already
+// exceeds the pruning threshold is discarded without touching its node.  Ties
+// in squared distance go to the lowest original triangle index, which
+// reproduces the reference's brute-force scan
 // (src/geometry/closest_point.cpp:50-68) exactly.
 #pragma once
 
@@ -40,13 +44,12 @@ __device__ __forceinline__ void cswap(float& ka, uint32_t& ra, int& sa, float& k
 
 constexpr int kSpill = 48;
 
-// sref / sd2 point at this lane's column of the LDS ring; consecutive slots
-// are `stride` elements apart.  thr_init bounds the search from above (a
+// sent points at this lane's column of the LDS ring (entry = node ref, bits
+// of the squared distance); consecutive slots are `stride` entries apart.  thr_init bounds the search from above (a
 // squared distance known to be >= the answer); FLT_MAX for an unbounded query.
 template <int CAP, bool COUNT>
 __device__ __forceinline__ Hit bvh4_closest(const Bvh4Node* __restrict__ nodes, const TriRec* __restrict__ tris,
-                                            V3 p, float delta, float thr_init, uint32_t* sref, float* sd2,
-                                            int stride) {
+                                            V3 p, float delta, float thr_init, uint2* sent, int stride) {
   static_assert((CAP & (CAP - 1)) == 0, "ring capacity must be a power of two");
   Hit h;
   h.d2 = FLT_MAX;
@@ -55,8 +58,7 @@ __device__ __forceinline__ Hit bvh4_closest(const Bvh4Node* __restrict__ nodes, 
   h.overflow = 0;
   h.nodes = 0;
   h.tris = 0;
-  uint32_t spill_ref[kSpill];
-  float spill_d2[kSpill];
+  uint64_t spill_e[kSpill];  // ref | distance bits << 32
   float thr = thr_init;
   uint32_t cur = 0;
   bool have = true;
@@ -93,8 +95,8 @@ __device__ __forceinline__ Hit bvh4_closest(const Bvh4Node* __restrict__ nodes, 
   do {                                                 \
     if (cnt == CAP) {                                  \
       if (scnt < kSpill) {                             \
-        spill_ref[scnt] = sref[top * stride];          \
-        spill_d2[scnt] = sd2[top * stride];            \
+        const uint2 ve_ = sent[top * stride];          \
+        spill_e[scnt] = (static_cast<uint64_t>(ve_.y) << 32) | ve_.x; \
         ++scnt;                                        \
       } else {                                         \
         h.overflow = 1;                                \
@@ -102,8 +104,7 @@ __device__ __forceinline__ Hit bvh4_closest(const Bvh4Node* __restrict__ nodes, 
     } else {                                           \
       ++cnt;                                           \
     }                                                  \
-    sref[top * stride] = (R);                          \
-    sd2[top * stride] = (K);                           \
+    sent[top * stride] = make_uint2((R), __float_as_uint(K)); \
     top = (top + 1) & (CAP - 1);                       \
   } while (0)
           if (nvalid >= 4) N2_PUSH(r3, k3);
@@ -147,12 +148,14 @@ __device__ __forceinline__ Hit bvh4_closest(const Bvh4Node* __restrict__ nodes, 
       if (cnt > 0) {
         top = (top - 1) & (CAP - 1);
         --cnt;
-        k = sd2[top * stride];
-        r = sref[top * stride];
+        const uint2 e = sent[top * stride];
+        k = __uint_as_float(e.y);
+        r = e.x;
       } else {
         --scnt;
-        k = spill_d2[scnt];
-        r = spill_ref[scnt];
+        const uint64_t e = spill_e[scnt];
+        k = __uint_as_float(static_cast<uint32_t>(e >> 32));
+        r = static_cast<uint32_t>(e);
       }
       if (k <= thr) {
         cur = r;

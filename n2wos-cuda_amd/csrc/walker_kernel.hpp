@@ -14,6 +14,18 @@ namespace n2 {
 #define N2_EMU_TRACE(sN, sT, sU)
 #endif
 
+// The walker's pruning threshold: prune_threshold with the hardware square
+// root and the relative margin raised from 2^-20 to 2^-19.  A square root at
+// most 2 ulp low still gives sqrt(best)(1 + 2^-19)(1 - 2^-22) > sqrt(best)(1 +
+// 2^-20), so this threshold is never below prune_threshold(best): the walker
+// visits a superset of the nodes the exact search needs, and its closest
+// points stay those of the brute-force scan.
+__device__ __forceinline__ float walker_prune_threshold(float best, float delta) {
+  if (best >= FLT_MAX) return FLT_MAX;
+  const float t = hw_sqrt(best) * 1.0000019073486328125f + delta;
+  return t * t;
+}
+
 
 // Chan et al. pairwise merge of (n, mean, M2); same expression order as the oracle.
 __host__ __device__ __forceinline__ void chan_merge(double na, double ma, double sa, double nb, double mb, double sb,
@@ -38,10 +50,11 @@ __host__ __device__ __forceinline__ void chan_merge(double na, double ma, double
 #endif
 constexpr int kWaves = N2_WAVES_PER_BLOCK;  // waves per block
 constexpr int kBlock = 64 * kWaves;
-// Waves per SIMD the register allocation must allow (the LDS ring of 16
-// entries per lane allows about 4 waves per SIMD, 8 entries about 6).
+// Waves per SIMD the register allocation must allow: 5 (at most 96 VGPRs,
+// no spills).  The LDS ring of 8 entries per lane (6.3 KB per one-wave
+// block) leaves room for more; a ring of 16 (10.4 KB) allows about 4.
 #ifndef N2_WALKER_WAVES_PER_SIMD
-#define N2_WALKER_WAVES_PER_SIMD 4
+#define N2_WALKER_WAVES_PER_SIMD 5
 #endif
 
 
@@ -65,8 +78,7 @@ constexpr int kUpdate = 3;  // query finished: take the walk step (or end the wa
 // node visits, triangle tests and walk steps is exactly the sequential one.
 template <int CAP, int S, bool POISSON, bool COUNT>
 __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persistent(WosKernelArgs a) {
-  __shared__ uint32_t s_ref[CAP * kBlock];
-  __shared__ float s_d2[CAP * kBlock];
+  __shared__ uint2 s_ent[CAP * kBlock];
   __shared__ float s_val[kWaves][S][64];
   __shared__ int s_unit[kWaves][S];
   __shared__ int s_rem[kWaves][S];
@@ -76,8 +88,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
 
   const int lane = lane_id();
   const int wv = static_cast<int>(threadIdx.x) >> 6;
-  uint32_t* sref = s_ref + threadIdx.x;
-  float* sd2 = s_d2 + threadIdx.x;
+  uint2* sent = s_ent + threadIdx.x;
   if (lane < S) s_unit[wv][lane] = -1;
   wave_sync();
 
@@ -97,8 +108,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
   int ti = 0, te = 0, top = 0, cnt = 0, scnt = 0, bid = -1, ovf = 0;
   float thr = FLT_MAX, bd2 = FLT_MAX;
   V3 bcp = v3(0.0f, 0.0f, 0.0f);
-  uint32_t spill_ref[kSpill];
-  float spill_d2[kSpill];
+  uint64_t spill_e[kSpill];
   uint32_t n_nodes = 0, n_tris = 0;
 
   // go to the entry `r` of the traversal
@@ -124,12 +134,14 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
       if (cnt > 0) {                                 \
         top = (top - 1) & (CAP - 1);                 \
         --cnt;                                       \
-        pop_key_ = sd2[top * kBlock];                \
-        pop_ref_ = sref[top * kBlock];               \
+        const uint2 le_ = sent[top * kBlock];        \
+        pop_key_ = __uint_as_float(le_.y);           \
+        pop_ref_ = le_.x;                            \
       } else {                                       \
         --scnt;                                      \
-        pop_key_ = spill_d2[scnt];                   \
-        pop_ref_ = spill_ref[scnt];                  \
+        const uint64_t se_ = spill_e[scnt];          \
+        pop_key_ = __uint_as_float(static_cast<uint32_t>(se_ >> 32)); \
+        pop_ref_ = static_cast<uint32_t>(se_);       \
       }                                              \
       if (pop_key_ <= thr) {                         \
         N2_ENTER(pop_ref_);                          \
@@ -143,8 +155,8 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
   do {                                               \
     if (cnt == CAP) {                                \
       if (scnt < kSpill) {                           \
-        spill_ref[scnt] = sref[top * kBlock];        \
-        spill_d2[scnt] = sd2[top * kBlock];          \
+        const uint2 ve_ = sent[top * kBlock];        \
+        spill_e[scnt] = (static_cast<uint64_t>(ve_.y) << 32) | ve_.x; \
         ++scnt;                                      \
       } else {                                       \
         ovf = 1;                                     \
@@ -152,8 +164,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
     } else {                                         \
       ++cnt;                                         \
     }                                                \
-    sref[top * kBlock] = (R);                        \
-    sd2[top * kBlock] = (K);                         \
+    sent[top * kBlock] = make_uint2((R), __float_as_uint(K)); \
     top = (top + 1) & (CAP - 1);                     \
   } while (0)
 
@@ -204,7 +215,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
       bd2 = d2;                                                                    \
       bcp = q;                                                                     \
       bid = id;                                                                    \
-      thr = fminf(thr, prune_threshold(d2, a.delta));                              \
+      thr = fminf(thr, walker_prune_threshold(d2, a.delta));                       \
     }                                                                              \
     ++ti;                                                                          \
     if (ti == te) N2_POP_NEXT();                                                   \
@@ -369,9 +380,12 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
             }
             p = v3(p.x + R * dir.x, p.y + R * dir.y, p.z + R * dir.z);
             ++step;
-            // |p_new - closest| <= d + |R dir|: bound the next query (loose
-            // margin covers FP32 rounding of d, R, dir and p)
-            const float tb = (d + R) * 1.0000152587890625f + 4.0f * a.delta;
+            // the previous closest point lies on the mesh, so the next
+            // distance is at most |p_new - closest| (about 1.3 d on average,
+            // against d + R): bound the next query by it.  The margin covers
+            // the FP32 rounding of the closest point and of the distance.
+            const V3 eo = p - bcp;
+            const float tb = hw_sqrt(dot(eo, eo)) * 1.0000152587890625f + 4.0f * a.delta;
             thr0 = tb * tb;
             bd2 = FLT_MAX;
             bcp = v3(0.0f, 0.0f, 0.0f);

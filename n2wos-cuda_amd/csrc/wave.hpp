@@ -24,6 +24,11 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
+
+// The hardware square root (V_SQRT_F32): one instruction, not correctly
+// rounded (within 1 ulp for normal inputs), against about 15 for sqrtf.
+// Only for bounds that carry a margin, never for values the results depend on.
+__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 #endif
 
 }  // namespace n2

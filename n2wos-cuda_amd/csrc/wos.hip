@@ -138,7 +138,7 @@ int wos_full_grid(int source) {
   N2_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   constexpr int kSlots = 8;
   const bool poisson = source == N2WOS_SOURCE_POISSON_SINES;
-  const int cap = tuning_value("ring", 16) == 16 ? 16 : 8;
+  const int cap = tuning_value("ring", 8) == 16 ? 16 : 8;
   const int per_cu = poisson ? variant_blocks_per_cu<kSlots, true>(cap) : variant_blocks_per_cu<kSlots, false>(cap);
   return cus * per_cu;
 }
@@ -191,7 +191,7 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
   launch_closest_point(bvh, r.d_points, r.n_points, first_d2, first_cp, first_tri, first_ovf, stream);
   constexpr int kSlots = 8;
   const bool poisson = r.source == N2WOS_SOURCE_POISSON_SINES;
-  const int cap = tuning_value("ring", 16);
+  const int cap = tuning_value("ring", 8);
   int grid = wos_full_grid(r.source);
   if (r.max_blocks > 0) grid = std::min(grid, r.max_blocks);
   const int64_t needed = (units + kWaves - 1) / kWaves;

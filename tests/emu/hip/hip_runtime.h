@@ -19,11 +19,25 @@ struct float4 {
 struct uint4 {
   uint32_t x, y, z, w;
 };
+struct uint2 {
+  uint32_t x, y;
+};
+inline uint2 make_uint2(uint32_t x, uint32_t y) { return uint2{x, y}; }
 
 inline int __float_as_int(float f) {
   int i;
   std::memcpy(&i, &f, 4);
   return i;
+}
+inline uint32_t __float_as_uint(float f) {
+  uint32_t i;
+  std::memcpy(&i, &f, 4);
+  return i;
+}
+inline float __uint_as_float(uint32_t i) {
+  float f;
+  std::memcpy(&f, &i, 4);
+  return f;
 }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline int __ffsll(unsigned long long x) { return __builtin_ffsll(static_cast<long long>(x)); }

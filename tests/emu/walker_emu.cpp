@@ -93,6 +93,7 @@ inline double __shfl_down(double v, int dlt) {
   }
 
 namespace n2 {
+inline float hw_sqrt(float x) { return std::sqrt(x); }
 inline int lane_id() { return emu::cur; }
 inline int rank_in(uint64_t mask) {
   const int l = emu::cur;
