@@ -212,6 +212,11 @@ typedef struct {
 
 /* Bytes of device workspace n2wos_wos_solve needs for this job. */
 int64_t n2wos_wos_workspace_size(int n_points, int walks_per_point);
+/* Blocks of the persistent walker's full-device grid on the current device
+ * for this source term (the launch that max_blocks = 0 asks for), so that a
+ * caller sharing the device can set max_blocks to a fraction of it.  No
+ * counterpart in the reference (its walkers launch one thread per walk). */
+int n2wos_wos_full_grid_blocks(int source, int* blocks);
 /* walks_per_point walks from each point; per-point FP64 (n, mean, M2),
  * total steps and forced-at-max-steps counts.  With accumulate != 0 the new
  * statistics are merged (Chan) into the existing contents of d_n, d_mean,

@@ -77,6 +77,7 @@ _SIGNATURES = {
     "n2wos_bvh_export": ([vp, vp, vp], i32),
     "n2wos_bvh_inside_device": ([vp, vp, i32, vp, vp], i32),
     "n2wos_wos_workspace_size": ([i32, i32], i64),
+    "n2wos_wos_full_grid_blocks": ([i32, vp], i32),
     "n2wos_cache_create": ([vp, C.POINTER(vp)], i32),
     "n2wos_cache_destroy": ([vp], i32),
     "n2wos_cache_infer": ([vp, vp, i32, vp, vp], i32),

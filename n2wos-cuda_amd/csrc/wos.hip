@@ -259,6 +259,16 @@ __global__ void k_nc_combine(int n_points, int wpp, int mode, const float* __res
 
 using namespace n2;
 
+extern "C" int n2wos_wos_full_grid_blocks(int source, int* blocks) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(blocks, N2WOS_ERR_NULL, "NULL output");
+    N2_REQUIRE(source == N2WOS_SOURCE_NONE || source == N2WOS_SOURCE_POISSON_SINES, N2WOS_ERR_INVALID,
+               "unknown source term");
+    *blocks = wos_full_grid(source);
+    return N2WOS_OK;
+  });
+}
+
 extern "C" int64_t n2wos_wos_workspace_size(int n_points, int walks_per_point) {
   if (n_points < 0 || walks_per_point < 1) return N2WOS_ERR_INVALID;
   return wos_workspace_bytes(n_points, walks_per_point);

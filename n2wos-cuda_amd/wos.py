@@ -93,6 +93,14 @@ class PointStats:
         return torch.where(self.n > 1, self.m2 / (self.n - 1).clamp(min=1), torch.zeros_like(self.m2))
 
 
+def full_grid_blocks(source: int = SOURCE_NONE) -> int:
+    """Blocks of the walker's full-device grid on the current device (what
+    max_blocks = 0 launches)."""
+    out = C.c_int(0)
+    check(lib().n2wos_wos_full_grid_blocks(source, C.byref(out)))
+    return out.value
+
+
 def workspace_bytes(n_points: int, walks_per_point: int) -> int:
     r = int(lib().n2wos_wos_workspace_size(n_points, walks_per_point))
     if r < 0:

@@ -146,3 +146,25 @@ def test_inside_test_against_radius(eng, torch_gpu, m1):
     rmin = np.linalg.norm(v, axis=1).min()
     assert np.all(ins[r < rmin * 0.99])
     assert not np.any(ins[r > 1.0])
+
+
+def test_partial_grid_equals_full_grid(eng, torch_gpu, m1):
+    """A walker limited to part of the device (max_blocks, as the progressive
+    estimator runs beside training) gives the full grid's results bit for
+    bit: units are claimed dynamically and reduced in a fixed order."""
+    from n2wos_amd import problems as P, wos as W
+    torch = torch_gpu
+    v, t, bvh, _ = m1
+    full = W.full_grid_blocks(W.SOURCE_NONE)
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    assert full > 0 and full % cus == 0
+    assert W.full_grid_blocks(W.SOURCE_POISSON_SINES) > 0
+    pts = P.interior_points(bvh, v, 2000, seed=9)
+    print("case: partial grid 2000x96, full grid %d blocks" % full, flush=True)
+    a = run(torch, bvh, P.charges_boundary(0), W.WalkParams(epsilon=1e-3, seed=5), pts, 96, per_walk=False)
+    for mb in (1, 7, full // 3):
+        b = run(torch, bvh, P.charges_boundary(0), W.WalkParams(epsilon=1e-3, seed=5, max_blocks=mb), pts, 96,
+                per_walk=False)
+        assert np.array_equal(a.mean.cpu().numpy().view(np.uint64), b.mean.cpu().numpy().view(np.uint64))
+        assert np.array_equal(a.m2.cpu().numpy().view(np.uint64), b.m2.cpu().numpy().view(np.uint64))
+        assert np.array_equal(a.steps.cpu().numpy(), b.steps.cpu().numpy())

@@ -19,8 +19,10 @@ def tri():
     srcs = [os.path.join(EMU, "tri_flat.cpp"), os.path.join(CSRC, "geom.hpp")]
     if not (os.path.exists(LIB) and os.path.getmtime(LIB) >= max(os.path.getmtime(s) for s in srcs)):
         os.makedirs(os.path.dirname(LIB), exist_ok=True)
+        tmp = f"{LIB}.{os.getpid()}.tmp"  # private name + rename: safe under parallel test processes
         subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-I" + EMU,
-                        "-I" + CSRC, os.path.join(EMU, "tri_flat.cpp"), "-o", LIB], check=True)
+                        "-I" + CSRC, os.path.join(EMU, "tri_flat.cpp"), "-o", tmp], check=True)
+        os.replace(tmp, LIB)
     L = C.CDLL(LIB)
     f = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
     L.tri_closest_both.argtypes = [f, f, f, f, C.c_long, f, f]

@@ -21,9 +21,13 @@ def _build():
     if os.path.exists(LIB) and os.path.getmtime(LIB) >= max(os.path.getmtime(s) for s in srcs):
         return
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    # build under a private name and rename, so parallel test processes never
+    # load a half-written library
+    tmp = f"{LIB}.{os.getpid()}.tmp"
     subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-I" + EMU, "-I" + CSRC,
                     "-I" + os.path.join(os.path.dirname(HERE), "include"), os.path.join(EMU, "walker_emu.cpp"),
-                    "-o", LIB], check=True)
+                    "-o", tmp], check=True)
+    os.replace(tmp, LIB)
 
 
 @pytest.fixture(scope="module")
