@@ -8,8 +8,8 @@
 // array (scratch memory, touched only by deep traversals).  Pops take the
 // ring first, then the spill array, so the visit order is exactly that of one
 // stack of CAP + kSpill entries.  A node visit leaves at most 3 entries
-// behind, so a tree of up to (CAP + kSpill) / 3 levels (18 for the walker's
-// ring of 8, 21 for the query kernel's 16) never overflows; the benchmark
+// behind, so a tree of up to (CAP + kSpill) / 3 levels (18 for a ring of 8,
+// 21 for a ring of 16) never overflows; the benchmark
 // meshes build 10 to 17 levels.  A deeper tree whose traversal fills the spill
 // array sets the overflow flag, like the reference's bounded stack.  Children
 // are visited nearest first; a popped entry whose stored distance already
@@ -117,9 +117,22 @@ __device__ __forceinline__ Hit bvh4_closest(const Bvh4Node* __restrict__ nodes, 
       } else {
         const int first = static_cast<int>((cur >> 5) & 0x3ffffffu);
         const int count = static_cast<int>(cur & 31u);
+        // the next record's loads are issued before this record's test, so
+        // the lane waits for memory once per leaf rather than per record
+        const float4* lp = reinterpret_cast<const float4*>(tris + first);
+        float4 n0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), n1 = n0, n2 = n0;
+        if (count > 0) {
+          n0 = lp[0];
+          n1 = lp[1];
+          n2 = lp[2];
+        }
         for (int i = 0; i < count; ++i) {
-          const float4* tp = reinterpret_cast<const float4*>(tris + first + i);
-          const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+          const float4 t0 = n0, t1 = n1, t2 = n2;
+          if (i + 1 < count) {
+            n0 = lp[3 * (i + 1)];
+            n1 = lp[3 * (i + 1) + 1];
+            n2 = lp[3 * (i + 1) + 2];
+          }
           if (COUNT) ++h.tris;
           // the triangle's own box bounds its distance from below
           const float bxl = fminf(fminf(t0.x, t0.w), t1.z), bxh = fmaxf(fmaxf(t0.x, t0.w), t1.z);

@@ -16,6 +16,7 @@
 struct float4 {
   float x, y, z, w;
 };
+inline float4 make_float4(float x, float y, float z, float w) { return float4{x, y, z, w}; }
 struct uint4 {
   uint32_t x, y, z, w;
 };
