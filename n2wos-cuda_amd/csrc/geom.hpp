@@ -274,6 +274,20 @@ N2_HD float complex_power_real(float x, float y, int k) {
   return ar;
 }
 
+// Sum over point charges of q / |p - c|, r^2 floored at 1e-12 (the
+// reference's charges boundary, src/cuda/cubql_bvh.cu:1264-1270); c holds
+// (cx, cy, cz, q) per charge.
+N2_HD float charges_value(const float* c, int n, V3 p) {
+  float value = 0.0f;
+  for (int i = 0; i < n; ++i) {
+    const float dx = p.x - c[4 * i];
+    const float dy = p.y - c[4 * i + 1];
+    const float dz = p.z - c[4 * i + 2];
+    value += c[4 * i + 3] / sqrtf(fmaxf(dx * dx + dy * dy + dz * dz, 1.0e-12f));
+  }
+  return value;
+}
+
 N2_HD float boundary_value(const Boundary& b, V3 p) {
   switch (b.mode) {
     case kBcX2MinusY2: return p.x * p.x - p.y * p.y;
@@ -281,16 +295,7 @@ N2_HD float boundary_value(const Boundary& b, V3 p) {
     case kBcZebraK8: return complex_power_real(p.x, p.y, 8);
     case kBcZebraK12: return complex_power_real(p.x, p.y, 12);
     case kBcConstantOne: return 1.0f;
-    case kBcCharges: {
-      float value = 0.0f;
-      for (int i = 0; i < b.n_charges; ++i) {
-        const float dx = p.x - b.charge[i][0];
-        const float dy = p.y - b.charge[i][1];
-        const float dz = p.z - b.charge[i][2];
-        value += b.charge[i][3] / sqrtf(fmaxf(dx * dx + dy * dy + dz * dz, 1.0e-12f));
-      }
-      return value;
-    }
+    case kBcCharges: return charges_value(&b.charge[0][0], b.n_charges, p);
     case kBcPoissonSines: {
       const float s = sin2pi(p.x) * sin2pi(p.y) * sin2pi(p.z);
       return s + 0.5f * (p.x * p.x + p.y * p.y + p.z * p.z);

@@ -14,6 +14,19 @@ namespace n2 {
 #define N2_EMU_TRACE(sN, sT, sU)
 #endif
 
+// Compare-exchange by key for the walker's child sort, branch-free.  Ties
+// keep no particular order: the visit order of equal-distance children
+// changes neither the closest point nor the walk.
+__device__ __forceinline__ void cswap_key(float& ka, uint32_t& ra, float& kb, uint32_t& rb) {
+  const bool sw = kb < ka;
+  const float lo = sw ? kb : ka, hi = sw ? ka : kb;
+  const uint32_t rlo = sw ? rb : ra, rhi = sw ? ra : rb;
+  ka = lo;
+  kb = hi;
+  ra = rlo;
+  rb = rhi;
+}
+
 // The walker's pruning threshold: prune_threshold with the hardware square
 // root and the relative margin raised from 2^-20 to 2^-19.  A square root at
 // most 2 ulp low still gives sqrt(best)(1 + 2^-19)(1 - 2^-22) > sqrt(best)(1 +
@@ -85,12 +98,14 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
   __shared__ uint32_t s_steps[kWaves][S];
   __shared__ uint32_t s_forced[kWaves][S];
   __shared__ uint32_t s_ovf[kWaves][S];
+  __shared__ float s_charge[4 * kMaxCharges];
 
   const int lane = lane_id();
   const int wv = static_cast<int>(threadIdx.x) >> 6;
   uint2* sent = s_ent + threadIdx.x;
   if (lane < S) s_unit[wv][lane] = -1;
-  wave_sync();
+  for (int i = static_cast<int>(threadIdx.x); i < 4 * kMaxCharges; i += kBlock) s_charge[i] = a.bc.charge[i >> 2][i & 3];
+  __syncthreads();
 
   // wave-uniform fill cursor
   int fill_slot = -1, fill_next = 0, fill_count = 0, fill_point = 0, fill_w0 = 0;
@@ -184,16 +199,15 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
       uint32_t r1 = static_cast<uint32_t>(__float_as_int(chf.y));                  \
       uint32_t r2 = static_cast<uint32_t>(__float_as_int(chf.z));                  \
       uint32_t r3 = static_cast<uint32_t>(__float_as_int(chf.w));                  \
-      int s0 = 0, s1 = 1, s2 = 2, s3 = 3;                                          \
       k0 = k0 <= thr ? k0 : INFINITY;                                              \
       k1 = k1 <= thr ? k1 : INFINITY;                                              \
       k2 = k2 <= thr ? k2 : INFINITY;                                              \
       k3 = k3 <= thr ? k3 : INFINITY;                                              \
-      cswap(k0, r0, s0, k1, r1, s1);                                               \
-      cswap(k2, r2, s2, k3, r3, s3);                                               \
-      cswap(k0, r0, s0, k2, r2, s2);                                               \
-      cswap(k1, r1, s1, k3, r3, s3);                                               \
-      cswap(k1, r1, s1, k2, r2, s2);                                               \
+      cswap_key(k0, r0, k1, r1);                                                   \
+      cswap_key(k2, r2, k3, r3);                                                   \
+      cswap_key(k0, r0, k2, r2);                                                   \
+      cswap_key(k1, r1, k3, r3);                                                   \
+      cswap_key(k1, r1, k2, r2);                                                   \
       if (nvalid >= 4) N2_PUSH(r3, k3);                                            \
       if (nvalid >= 3) N2_PUSH(r2, k2);                                            \
       if (nvalid >= 2) N2_PUSH(r1, k1);                                            \
@@ -323,7 +337,21 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
           // reference's hybrid prefix does, src/cuda/cubql_bvh.cu:1486-1510)
           const bool stop_prefix = !at_boundary && a.mode == kModePrefix && step >= a.depth;
           if (at_boundary || stop_prefix) {
-            const float value = at_boundary ? src + boundary_value(a.bc, bcp) : src;
+            float gval = 0.0f;
+            if (at_boundary) {
+              if (a.bc.mode == kBcCharges) {
+                // the charges are read from LDS at an offset the compiler
+                // cannot see through, so their 64 values are not held in
+                // registers across the loop
+                gval = charges_value(s_charge + opaque_lane_int(0), a.bc.n_charges, bcp);
+              } else {
+                Boundary b0;
+                b0.mode = a.bc.mode;
+                b0.n_charges = 0;
+                gval = boundary_value(b0, bcp);
+              }
+            }
+            const float value = at_boundary ? src + gval : src;
             if (a.mode != kModePlain) {
               const int64_t sidx = static_cast<int64_t>(point) * a.walks_per_point + walk;
               if (a.mode == kModeCoupled) a.nc_full[sidx] = value;
@@ -363,14 +391,18 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
             const bool anti = a.antithetic && step == 0 && (walk & 1);
             const uint32_t cw = anti ? ctr_walk - 1u : ctr_walk;
             const float sgn = anti ? -1.0f : 1.0f;
+            // the key words pass through an opaque copy, so the compiler
+            // derives the ten round keys here (scalar adds) instead of
+            // keeping 20 of them in registers across the whole loop
+            const uint32_t key_lo = opaque_u32(a.seed_lo), key_hi = opaque_u32(a.seed_hi);
             const U4 rr = philox4x32_10(U4{static_cast<uint32_t>(step), cw, ctr_point, 2u * a.stream},
-                                        a.seed_lo, a.seed_hi);
+                                        key_lo, key_hi);
             V3 dir = sphere_dir(rr.x, rr.y);
             dir = v3(sgn * dir.x, sgn * dir.y, sgn * dir.z);
             const float R = a.step_scale * d;
             if (POISSON) {
               const U4 t = philox4x32_10(U4{static_cast<uint32_t>(step), cw, ctr_point, 2u * a.stream + 1u},
-                                         a.seed_lo, a.seed_hi);
+                                         key_lo, key_hi);
               const float rho = median3(u01(rr.z), u01(rr.w), u01(t.x));
               V3 e = sphere_dir(t.y, t.z);
               e = v3(sgn * e.x, sgn * e.y, sgn * e.z);

@@ -29,6 +29,18 @@ __device__ __forceinline__ void wave_sync() {
 // rounded (within 1 ulp for normal inputs), against about 15 for sqrtf.
 // Only for bounds that carry a margin, never for values the results depend on.
 __device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// The same value, through a copy the compiler cannot see through: a use
+// after it cannot be hoisted out of a loop, so a loop-invariant value that is
+// needed only on a rare path is not held in registers for the whole loop.
+__device__ __forceinline__ uint32_t opaque_u32(uint32_t v) {
+  asm volatile("" : "+s"(v));
+  return v;
+}
+__device__ __forceinline__ int opaque_lane_int(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
 #endif
 
 }  // namespace n2

@@ -94,6 +94,8 @@ inline double __shfl_down(double v, int dlt) {
 
 namespace n2 {
 inline float hw_sqrt(float x) { return std::sqrt(x); }
+inline uint32_t opaque_u32(uint32_t v) { return v; }
+inline int opaque_lane_int(int v) { return v; }
 inline int lane_id() { return emu::cur; }
 inline int rank_in(uint64_t mask) {
   const int l = emu::cur;
@@ -101,6 +103,8 @@ inline int rank_in(uint64_t mask) {
 }
 inline void wave_sync() { emu::arrive(emu::kSync); }
 }  // namespace n2
+// the emulated block is one wave
+inline void __syncthreads() { n2::wave_sync(); }
 
 #include "../../n2wos-cuda_amd/csrc/walker_kernel.hpp"
 
