@@ -100,6 +100,10 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
   __shared__ uint32_t s_ovf[kWaves][S];
   __shared__ float s_charge[4 * kMaxCharges];
 
+  // arguments used only on rare paths (refill, walk step, walk end) are read
+  // where they are used instead of being held in scalar registers across the
+  // loop, which the node and triangle paths need
+#define KA N2_COLD_ARGS(a)
   const int lane = lane_id();
   const int wv = static_cast<int>(threadIdx.x) >> 6;
   uint2* sent = s_ent + threadIdx.x;
@@ -252,15 +256,15 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
           }
           if (fs < 0) break;
           int u = 0;
-          if (lane == 0) u = atomicAdd(a.next_unit, 1);
+          if (lane == 0) u = atomicAdd(KA->next_unit, 1);
           u = __shfl(u, 0);
-          if (u >= a.total_units) {
+          if (u >= KA->total_units) {
             no_more = true;
             break;
           }
-          const int pt = u / a.units_per_point;
-          const int chunk = u - pt * a.units_per_point;
-          const int ucnt = min(64, a.walks_per_point - 64 * chunk);
+          const int pt = u / KA->units_per_point;
+          const int chunk = u - pt * KA->units_per_point;
+          const int ucnt = min(64, KA->walks_per_point - 64 * chunk);
           if (lane == 0) {
             s_unit[wv][fs] = u;
             s_rem[wv][fs] = ucnt;
@@ -284,21 +288,21 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
           wiu = fill_next + r;
           point = fill_point;
           walk = fill_w0 + wiu;
-          if (a.point_rng) {
-            ctr_point = a.point_rng[2 * point];
-            ctr_walk = a.point_rng[2 * point + 1] + static_cast<uint32_t>(walk);
+          if (KA->point_rng) {
+            ctr_point = KA->point_rng[2 * point];
+            ctr_walk = KA->point_rng[2 * point + 1] + static_cast<uint32_t>(walk);
           } else {
-            ctr_point = static_cast<uint32_t>(point) + a.point_offset;
+            ctr_point = static_cast<uint32_t>(point) + KA->point_offset;
             ctr_walk = static_cast<uint32_t>(walk);
           }
           step = 0;
           src = 0.0f;
           thr0 = FLT_MAX;
-          p = v3(a.points[3 * point], a.points[3 * point + 1], a.points[3 * point + 2]);
-          bd2 = a.first_d2[point];
-          bcp = v3(a.first_cp[3 * point], a.first_cp[3 * point + 1], a.first_cp[3 * point + 2]);
+          p = v3(KA->points[3 * point], KA->points[3 * point + 1], KA->points[3 * point + 2]);
+          bd2 = KA->first_d2[point];
+          bcp = v3(KA->first_cp[3 * point], KA->first_cp[3 * point + 1], KA->first_cp[3 * point + 2]);
           bid = 0;
-          ovf = a.first_ovf[point];
+          ovf = KA->first_ovf[point];
           phase = kUpdate;
         }
         fill_next += take;
@@ -331,77 +335,77 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
           phase = kNode;
         } else {
           const float d = sqrtf(fmaxf(bd2, 0.0f));
-          const bool forced = step >= a.max_steps;
-          const bool at_boundary = d <= a.epsilon || forced || !isfinite(d);
+          const bool forced = step >= KA->max_steps;
+          const bool at_boundary = d <= KA->epsilon || forced || !isfinite(d);
           // prefix mode ends a walk at depth m (after the boundary test, as the
           // reference's hybrid prefix does, src/cuda/cubql_bvh.cu:1486-1510)
-          const bool stop_prefix = !at_boundary && a.mode == kModePrefix && step >= a.depth;
+          const bool stop_prefix = !at_boundary && KA->mode == kModePrefix && step >= KA->depth;
           if (at_boundary || stop_prefix) {
             float gval = 0.0f;
             if (at_boundary) {
-              if (a.bc.mode == kBcCharges) {
+              if (KA->bc.mode == kBcCharges) {
                 // the charges are read from LDS at an offset the compiler
                 // cannot see through, so their 64 values are not held in
                 // registers across the loop
-                gval = charges_value(s_charge + opaque_lane_int(0), a.bc.n_charges, bcp);
+                gval = charges_value(s_charge + opaque_lane_int(0), KA->bc.n_charges, bcp);
               } else {
                 Boundary b0;
-                b0.mode = a.bc.mode;
+                b0.mode = KA->bc.mode;
                 b0.n_charges = 0;
                 gval = boundary_value(b0, bcp);
               }
             }
             const float value = at_boundary ? src + gval : src;
-            if (a.mode != kModePlain) {
-              const int64_t sidx = static_cast<int64_t>(point) * a.walks_per_point + walk;
-              if (a.mode == kModeCoupled) a.nc_full[sidx] = value;
+            if (KA->mode != kModePlain) {
+              const int64_t sidx = static_cast<int64_t>(point) * KA->walks_per_point + walk;
+              if (KA->mode == kModeCoupled) KA->nc_full[sidx] = value;
               // coupled walks that passed depth m recorded their prefix there
-              if (a.mode == kModePrefix || step <= a.depth) {
-                a.nc_prefix[sidx] = value;
-                a.nc_need[sidx] = stop_prefix ? 1 : 0;
-                a.nc_xm[3 * sidx] = p.x;
-                a.nc_xm[3 * sidx + 1] = p.y;
-                a.nc_xm[3 * sidx + 2] = p.z;
+              if (KA->mode == kModePrefix || step <= KA->depth) {
+                KA->nc_prefix[sidx] = value;
+                KA->nc_need[sidx] = stop_prefix ? 1 : 0;
+                KA->nc_xm[3 * sidx] = p.x;
+                KA->nc_xm[3 * sidx + 1] = p.y;
+                KA->nc_xm[3 * sidx + 2] = p.z;
               }
             }
             s_val[wv][slot][wiu] = value;
             atomicAdd(&s_steps[wv][slot], static_cast<uint32_t>(step));
             if (forced) atomicAdd(&s_forced[wv][slot], 1u);
             if (ovf) atomicAdd(&s_ovf[wv][slot], 1u);
-            if (a.walk_values) {
-              const int64_t sidx = static_cast<int64_t>(point) * a.walks_per_point + walk;
-              a.walk_values[sidx] = value;
-              a.walk_steps[sidx] = step;
+            if (KA->walk_values) {
+              const int64_t sidx = static_cast<int64_t>(point) * KA->walks_per_point + walk;
+              KA->walk_values[sidx] = value;
+              KA->walk_steps[sidx] = step;
             }
             const int old = atomicSub(&s_rem[wv][slot], 1);
             unit_done = (old == 1);
             phase = kIdle;
           } else {
-            if (a.mode == kModeCoupled && step == a.depth) {
+            if (KA->mode == kModeCoupled && step == KA->depth) {
               // the coupled walk's NC state at depth m: prefix source and X_m
-              const int64_t sidx = static_cast<int64_t>(point) * a.walks_per_point + walk;
-              a.nc_prefix[sidx] = src;
-              a.nc_need[sidx] = 1;
-              a.nc_xm[3 * sidx] = p.x;
-              a.nc_xm[3 * sidx + 1] = p.y;
-              a.nc_xm[3 * sidx + 2] = p.z;
+              const int64_t sidx = static_cast<int64_t>(point) * KA->walks_per_point + walk;
+              KA->nc_prefix[sidx] = src;
+              KA->nc_need[sidx] = 1;
+              KA->nc_xm[3 * sidx] = p.x;
+              KA->nc_xm[3 * sidx + 1] = p.y;
+              KA->nc_xm[3 * sidx + 2] = p.z;
             }
             // antithetic pairs: the odd walk of a pair takes its first step
             // along the negated direction drawn for its even partner
-            const bool anti = a.antithetic && step == 0 && (walk & 1);
+            const bool anti = KA->antithetic && step == 0 && (walk & 1);
             const uint32_t cw = anti ? ctr_walk - 1u : ctr_walk;
             const float sgn = anti ? -1.0f : 1.0f;
-            // the key words pass through an opaque copy, so the compiler
-            // derives the ten round keys here (scalar adds) instead of
-            // keeping 20 of them in registers across the whole loop
-            const uint32_t key_lo = opaque_u32(a.seed_lo), key_hi = opaque_u32(a.seed_hi);
-            const U4 rr = philox4x32_10(U4{static_cast<uint32_t>(step), cw, ctr_point, 2u * a.stream},
+            // the key words are read here, so the compiler derives the ten
+            // round keys here (scalar adds) instead of keeping 20 of them in
+            // registers across the whole loop
+            const uint32_t key_lo = KA->seed_lo, key_hi = KA->seed_hi;
+            const U4 rr = philox4x32_10(U4{static_cast<uint32_t>(step), cw, ctr_point, 2u * KA->stream},
                                         key_lo, key_hi);
             V3 dir = sphere_dir(rr.x, rr.y);
             dir = v3(sgn * dir.x, sgn * dir.y, sgn * dir.z);
-            const float R = a.step_scale * d;
+            const float R = KA->step_scale * d;
             if (POISSON) {
-              const U4 t = philox4x32_10(U4{static_cast<uint32_t>(step), cw, ctr_point, 2u * a.stream + 1u},
+              const U4 t = philox4x32_10(U4{static_cast<uint32_t>(step), cw, ctr_point, 2u * KA->stream + 1u},
                                          key_lo, key_hi);
               const float rho = median3(u01(rr.z), u01(rr.w), u01(t.x));
               V3 e = sphere_dir(t.y, t.z);
@@ -442,8 +446,8 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
         done &= done - 1;
         const int s = __shfl(slot, l);
         const int u = s_unit[wv][s];
-        const int pt = u / a.units_per_point;
-        const int ucnt = min(64, a.walks_per_point - 64 * (u - pt * a.units_per_point));
+        const int pt = u / KA->units_per_point;
+        const int ucnt = min(64, KA->walks_per_point - 64 * (u - pt * KA->units_per_point));
         double n = lane < ucnt ? 1.0 : 0.0;
         double m = lane < ucnt ? static_cast<double>(s_val[wv][s][lane]) : 0.0;
         double q = 0.0;
@@ -455,10 +459,10 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
           if ((lane & (2 * stride - 1)) == 0) chan_merge(n, m, q, nb, mb, qb, n, m, q);
         }
         if (lane == 0) {
-          a.unit_mean[u] = m;
-          a.unit_m2[u] = q;
-          a.unit_steps[u] = s_steps[wv][s];
-          a.unit_flags[u] = (s_forced[wv][s] & 0xffffu) | (s_ovf[wv][s] << 16);
+          KA->unit_mean[u] = m;
+          KA->unit_m2[u] = q;
+          KA->unit_steps[u] = s_steps[wv][s];
+          KA->unit_flags[u] = (s_forced[wv][s] & 0xffffu) | (s_ovf[wv][s] << 16);
           s_unit[wv][s] = -1;
         }
         wave_sync();
@@ -479,6 +483,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
       }
     }
   }
+#undef KA
 #undef N2_ENTER
 #undef N2_POP_NEXT
 #undef N2_PUSH

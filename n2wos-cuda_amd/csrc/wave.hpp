@@ -41,6 +41,19 @@ __device__ __forceinline__ int opaque_lane_int(int v) {
   asm volatile("" : "+v"(v));
   return v;
 }
+
+// A kernel's by-value argument block, read in place from the kernel-argument
+// segment (it starts there) through an opaque pointer: each use is a scalar
+// load where it stands, for arguments needed only on rare paths.
+template <class T>
+using kernarg_ptr = const T __attribute__((address_space(4)))*;
+template <class T>
+__device__ __forceinline__ kernarg_ptr<T> kernel_args() {
+  kernarg_ptr<T> p = (kernarg_ptr<T>)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(p));
+  return p;
+}
+#define N2_COLD_ARGS(a) (::n2::kernel_args<WosKernelArgs>())
 #endif
 
 }  // namespace n2
