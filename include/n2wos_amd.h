@@ -69,7 +69,8 @@ int n2wos_abi_version(void);
  * per lane of the walker (8 or 16, default 8); "refill_min" = idle lanes
  * that trigger a refill (1..64, default 2); "weight_node" / "weight_tri" /
  * "weight_update" = per-lane weights of the walker's phase choice (10, 10 and
- * 30). */
+ * 30); "query_quad" = 1 (default): closest-point queries run on four lanes
+ * each, 0: one lane per query with the caller's block size (same results). */
 int n2wos_set_tuning(const char* key, int value);
 /* Device name and runtime version of the current HIP device, for reports. */
 int n2wos_device_summary(char* buf, int buf_len);

@@ -49,6 +49,42 @@ __global__ void __launch_bounds__(1024) k_closest_point(const Bvh4Node* __restri
   if (COUNT) { out_nodes[q] = h.nodes; out_tris[q] = h.tris; }
 }
 
+// Four lanes per query (see bvh4_closest_quad): block of kQuadBlock lanes =
+// kQuadBlock / 4 queries; each lane keeps its own copy of the query's stack
+// in an LDS ring of kQuadCap entries.
+constexpr int kQuadBlock = 256;
+constexpr int kQuadCap = 8;
+
+template <bool MASKED, bool COUNT>
+__global__ void __launch_bounds__(kQuadBlock) k_closest_point_quad(
+    const Bvh4Node* __restrict__ nodes, const TriRec* __restrict__ tris, float delta,
+    const float* __restrict__ points, const uint8_t* __restrict__ active, int n, float* __restrict__ out_d2,
+    float* __restrict__ out_cp, int* __restrict__ out_tri, int* __restrict__ out_overflow,
+    uint32_t* __restrict__ out_nodes, uint32_t* __restrict__ out_tris) {
+  __shared__ uint2 s_ring[kQuadCap * kQuadBlock];
+  const int t = static_cast<int>(threadIdx.x);
+  const int k = t & 3;
+  const int q = static_cast<int>((static_cast<int64_t>(blockIdx.x) * kQuadBlock + t) >> 2);
+  if (q >= n) return;  // all four lanes of a quad share q
+  const V3 p = v3(points[3 * q], points[3 * q + 1], points[3 * q + 2]);
+  Hit h;
+  if (MASKED && !active[q]) {
+    // inactive slot: neutral outputs (src/cuda/cubql_bvh.cu:198-204)
+    h.d2 = 0.0f; h.cp = p; h.id = -1; h.overflow = 0; h.nodes = 0; h.tris = 0;
+  } else if (finite3(p)) {
+    h = bvh4_closest_quad<kQuadCap, COUNT>(nodes, tris, p, delta, s_ring + t, kQuadBlock, k);
+  } else {
+    h.d2 = FLT_MAX; h.cp = v3(0.0f, 0.0f, 0.0f); h.id = -1; h.overflow = 0; h.nodes = 0; h.tris = 0;
+  }
+  if (k == 0) {
+    out_d2[q] = h.d2;
+    out_cp[3 * q] = h.cp.x; out_cp[3 * q + 1] = h.cp.y; out_cp[3 * q + 2] = h.cp.z;
+    out_tri[q] = h.id;
+    out_overflow[q] = h.overflow;
+    if (COUNT) { out_nodes[q] = h.nodes; out_tris[q] = h.tris; }
+  }
+}
+
 // Ray-parity inside test (Moller-Trumbore hits with t > 0 along three fixed
 // generic directions; majority vote), one lane per point.
 __device__ int ray_crossings(const Bvh4Node* __restrict__ nodes, const TriRec* __restrict__ tris, V3 o, V3 dir) {
@@ -123,6 +159,17 @@ int launch_query(const n2wos_bvh* bvh, const float* d_points, const uint8_t* d_a
   N2_REQUIRE(!MASKED || d_active, N2WOS_ERR_NULL, "query_device_masked received a null mask");
   N2_REQUIRE(!COUNT || (d_nodes && d_tris), N2WOS_ERR_NULL, "counted query received a null counter pointer");
   N2_REQUIRE(block >= 1 && block <= 1024, N2WOS_ERR_BLOCK, "block size must be in [1, 1024]");
+  if (tuning_value("query_quad", 1)) {
+    // four lanes per query with its own launch shape; block_size, validated
+    // above as the reference does, changes no result
+    const int64_t lanes = 4 * static_cast<int64_t>(n);
+    const int grid = static_cast<int>((lanes + kQuadBlock - 1) / kQuadBlock);
+    hipLaunchKernelGGL((k_closest_point_quad<MASKED, COUNT>), dim3(grid), dim3(kQuadBlock), 0,
+                       static_cast<hipStream_t>(stream), bvh->d_nodes, bvh->d_tris, bvh->delta, d_points, d_active, n,
+                       d_d2, d_cp, d_tri, d_overflow, d_nodes, d_tris);
+    N2_HIP(hipGetLastError());
+    return N2WOS_OK;
+  }
   const size_t lds = static_cast<size_t>(kQueryCap) * static_cast<size_t>(block) * sizeof(uint2);
   auto kern = k_closest_point<MASKED, COUNT>;
   if (lds > 65536) {

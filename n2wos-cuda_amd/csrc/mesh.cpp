@@ -110,9 +110,11 @@ extern "C" int n2wos_set_tuning(const char* key, int value) {
   return guarded([&]() -> int {
     N2_REQUIRE(key != nullptr, N2WOS_ERR_NULL, "NULL key");
     const std::string k(key);
-    N2_REQUIRE(k == "ring" || k == "refill_min" || k == "weight_node" || k == "weight_tri" || k == "weight_update",
+    N2_REQUIRE(k == "ring" || k == "refill_min" || k == "weight_node" || k == "weight_tri" || k == "weight_update" ||
+                   k == "query_quad",
                N2WOS_ERR_INVALID, "unknown tuning key: " + k);
     if (k == "ring") N2_REQUIRE(value == 8 || value == 16, N2WOS_ERR_INVALID, "ring must be 8 or 16");
+    if (k == "query_quad") N2_REQUIRE(value == 0 || value == 1, N2WOS_ERR_INVALID, "query_quad must be 0 or 1");
     if (k == "refill_min") N2_REQUIRE(value >= 1 && value <= 64, N2WOS_ERR_INVALID, "refill_min must be in [1, 64]");
     if (k.rfind("weight_", 0) == 0) N2_REQUIRE(value >= 1 && value <= 1000, N2WOS_ERR_INVALID, "weight in [1, 1000]");
     std::lock_guard<std::mutex> lock(g_tuning_mu);

@@ -181,4 +181,169 @@ __device__ __forceinline__ Hit bvh4_closest(const Bvh4Node* __restrict__ nodes, 
   return h;
 }
 
+// ---------------------------------------------------------------- quads
+// Four lanes (one quad: lanes 4j .. 4j+3, which DPP quad permutations reach
+// in one instruction) share one query.  Lane k of the quad loads and tests
+// child k of each node and triangle k, k+4, ... of each leaf; every lane
+// holds the same copy of the query's state (stack, threshold, best), so the
+// quad's control flow is uniform and the visit order, the nodes and
+// triangle records visited and the result are exactly those of
+// bvh4_closest.  A wave serves 16 queries, so a far query keeps 4 lanes
+// busy instead of holding up 63 idle ones.
+#ifndef N2_EMULATED_WAVE
+template <int J>
+__device__ __forceinline__ uint32_t quad_bcast(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), J * 0x55, 0xF, 0xF, false));
+}
+template <int J>
+__device__ __forceinline__ float quad_bcast(float v) {
+  return __uint_as_float(quad_bcast<J>(__float_as_uint(v)));
+}
+// value of the quad lane whose index is this lane's XOR X (X = 1 or 2)
+template <int X>
+__device__ __forceinline__ uint32_t quad_xor(uint32_t v) {
+  constexpr int ctrl = X == 1 ? 0xB1 : 0x4E;  // quad_perm [1,0,3,2] / [2,3,0,1]
+  return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), ctrl, 0xF, 0xF, false));
+}
+
+// (d2, id) lexicographic minimum over the quad, carrying the closest point
+template <int X>
+__device__ __forceinline__ void quad_min_step(float& d2, int& id, V3& q) {
+  const float od2 = __uint_as_float(quad_xor<X>(__float_as_uint(d2)));
+  const int oid = static_cast<int>(quad_xor<X>(static_cast<uint32_t>(id)));
+  const float ox = __uint_as_float(quad_xor<X>(__float_as_uint(q.x)));
+  const float oy = __uint_as_float(quad_xor<X>(__float_as_uint(q.y)));
+  const float oz = __uint_as_float(quad_xor<X>(__float_as_uint(q.z)));
+  const bool take = od2 < d2 || (od2 == d2 && oid < id);
+  d2 = take ? od2 : d2;
+  id = take ? oid : id;
+  q = v3(take ? ox : q.x, take ? oy : q.y, take ? oz : q.z);
+}
+
+template <int CAP, bool COUNT>
+__device__ __forceinline__ Hit bvh4_closest_quad(const Bvh4Node* __restrict__ nodes,
+                                                 const TriRec* __restrict__ tris, V3 p, float delta, uint2* sent,
+                                                 int stride, int k) {
+  static_assert((CAP & (CAP - 1)) == 0, "ring capacity must be a power of two");
+  Hit h;
+  h.d2 = FLT_MAX;
+  h.cp = v3(0.0f, 0.0f, 0.0f);
+  h.id = -1;
+  h.overflow = 0;
+  h.nodes = 0;
+  h.tris = 0;
+  uint64_t spill_e[kSpill];  // ref | distance bits << 32
+  float thr = FLT_MAX;
+  uint32_t cur = 0;
+  bool have = true;
+  int top = 0, cnt = 0, scnt = 0;
+  while (true) {
+    if (have) {
+      if (!(cur & kLeafFlag)) {
+        const Bvh4Node* nd = nodes + cur;
+        if (COUNT) ++h.nodes;
+        const float kk = box_dist2(p, nd->lox[k], nd->loy[k], nd->loz[k], nd->hix[k], nd->hiy[k], nd->hiz[k]);
+        const uint32_t rk = nd->child[k];
+        float k0 = quad_bcast<0>(kk), k1 = quad_bcast<1>(kk), k2 = quad_bcast<2>(kk), k3 = quad_bcast<3>(kk);
+        uint32_t r0 = quad_bcast<0>(rk), r1 = quad_bcast<1>(rk), r2 = quad_bcast<2>(rk), r3 = quad_bcast<3>(rk);
+        const int nvalid = (k0 <= thr) + (k1 <= thr) + (k2 <= thr) + (k3 <= thr);
+        if (nvalid == 0) {
+          have = false;
+        } else {
+          int s0 = 0, s1 = 1, s2 = 2, s3 = 3;
+          k0 = k0 <= thr ? k0 : INFINITY;
+          k1 = k1 <= thr ? k1 : INFINITY;
+          k2 = k2 <= thr ? k2 : INFINITY;
+          k3 = k3 <= thr ? k3 : INFINITY;
+          cswap(k0, r0, s0, k1, r1, s1);
+          cswap(k2, r2, s2, k3, r3, s3);
+          cswap(k0, r0, s0, k2, r2, s2);
+          cswap(k1, r1, s1, k3, r3, s3);
+          cswap(k1, r1, s1, k2, r2, s2);
+#define N2_PUSH(R, K)                                  \
+  do {                                                 \
+    if (cnt == CAP) {                                  \
+      if (scnt < kSpill) {                             \
+        const uint2 ve_ = sent[top * stride];          \
+        spill_e[scnt] = (static_cast<uint64_t>(ve_.y) << 32) | ve_.x; \
+        ++scnt;                                        \
+      } else {                                         \
+        h.overflow = 1;                                \
+      }                                                \
+    } else {                                           \
+      ++cnt;                                           \
+    }                                                  \
+    sent[top * stride] = make_uint2((R), __float_as_uint(K)); \
+    top = (top + 1) & (CAP - 1);                       \
+  } while (0)
+          if (nvalid >= 4) N2_PUSH(r3, k3);
+          if (nvalid >= 3) N2_PUSH(r2, k2);
+          if (nvalid >= 2) N2_PUSH(r1, k1);
+#undef N2_PUSH
+          cur = r0;
+          continue;
+        }
+      } else {
+        const int first = static_cast<int>((cur >> 5) & 0x3ffffffu);
+        const int count = static_cast<int>(cur & 31u);
+        if (COUNT) h.tris += static_cast<uint32_t>(count);
+        for (int i0 = 0; i0 < count; i0 += 4) {
+          float d2 = INFINITY;
+          int id = 0x7fffffff;
+          V3 q = v3(0.0f, 0.0f, 0.0f);
+          if (i0 + k < count) {
+            const float4* tp = reinterpret_cast<const float4*>(tris + first + i0 + k);
+            const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+            // the triangle's own box bounds its distance from below
+            const float bxl = fminf(fminf(t0.x, t0.w), t1.z), bxh = fmaxf(fmaxf(t0.x, t0.w), t1.z);
+            const float byl = fminf(fminf(t0.y, t1.x), t1.w), byh = fmaxf(fmaxf(t0.y, t1.x), t1.w);
+            const float bzl = fminf(fminf(t0.z, t1.y), t2.x), bzh = fmaxf(fmaxf(t0.z, t1.y), t2.x);
+            if (box_dist2(p, bxl, byl, bzl, bxh, byh, bzh) <= thr) {
+              const V3 a = v3(t0.x, t0.y, t0.z), b = v3(t0.w, t1.x, t1.y), c = v3(t1.z, t1.w, t2.x);
+              q = closest_on_triangle_flat(p, a, b, c);
+              const V3 e = q - p;
+              d2 = dot(e, e);
+              id = __float_as_int(t2.y);
+            }
+          }
+          quad_min_step<1>(d2, id, q);
+          quad_min_step<2>(d2, id, q);
+          if (d2 < h.d2 || (d2 == h.d2 && id < h.id)) {
+            h.d2 = d2;
+            h.cp = q;
+            h.id = id;
+            thr = fminf(thr, prune_threshold(d2, delta));
+          }
+        }
+        have = false;
+      }
+    }
+    // pop the next entry that can still hold a closer triangle
+    while (cnt > 0 || scnt > 0) {
+      float kp;
+      uint32_t r;
+      if (cnt > 0) {
+        top = (top - 1) & (CAP - 1);
+        --cnt;
+        const uint2 e = sent[top * stride];
+        kp = __uint_as_float(e.y);
+        r = e.x;
+      } else {
+        --scnt;
+        const uint64_t e = spill_e[scnt];
+        kp = __uint_as_float(static_cast<uint32_t>(e >> 32));
+        r = static_cast<uint32_t>(e);
+      }
+      if (kp <= thr) {
+        cur = r;
+        have = true;
+        break;
+      }
+    }
+    if (!have) break;
+  }
+  return h;
+}
+#endif
+
 }  // namespace n2

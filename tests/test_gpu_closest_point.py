@@ -128,3 +128,36 @@ def test_sah_build_bit_exact_vs_bruteforce(eng, orc, torch_gpu, mesh, leaf):
     d2b, cpb, trb = orc.bruteforce(v, t, pts, nthreads=16)
     assert np.array_equal(bits(d2), bits(d2b)) and np.array_equal(tri, trb) and np.array_equal(bits(cp), bits(cpb))
     assert ovf.sum() == 0
+
+
+def test_quad_and_per_lane_kernels_agree(eng, torch_gpu, c1):
+    """The four-lanes-per-query kernel (default) and the one-lane-per-query
+    kernel return the same outputs and visit counts for every query, masked
+    or not, including non-finite points, far points and a count that is not
+    a multiple of the block."""
+    import n2wos_amd._lib as L
+    vn, t, bvh = c1
+    torch = torch_gpu
+    rng = np.random.default_rng(7)
+    pts = np.concatenate([rng.uniform(-1.2, 1.2, (3000, 3)), rng.uniform(-40, 40, (200, 3)),
+                          vn[rng.integers(0, len(vn), 301)] + rng.normal(0, 1e-3, (301, 3))]).astype(np.float32)
+    pts[17] = [np.nan, 0, 0]
+    pts[1030] = [0, -np.inf, 0]
+    active = (rng.uniform(size=len(pts)) < 0.7).astype(np.uint8)
+    tp = torch.from_numpy(pts).cuda()
+    ta = torch.from_numpy(active).cuda()
+    res = {}
+    try:
+        for quad in (1, 0):
+            L.check(L.lib().n2wos_set_tuning(b"query_quad", quad))
+            print("case: query_quad=%d, %d points" % (quad, len(pts)), flush=True)
+            plain = [x.cpu().numpy() for x in bvh.query_tensors(tp, counted=True)]
+            masked = [x.cpu().numpy() for x in bvh.query_tensors(tp, active=ta)]
+            torch.cuda.synchronize()
+            res[quad] = plain + masked
+    finally:
+        L.check(L.lib().n2wos_set_tuning(b"query_quad", 1))
+    for a, b in zip(res[1], res[0]):
+        assert np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32))
+    d2, cp, tri, ovf, nn, nt = res[1][:6]
+    assert tri[17] == -1 and tri[1030] == -1 and ovf.sum() == 0 and nn.sum() > 0
