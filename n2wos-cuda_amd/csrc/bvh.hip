@@ -50,10 +50,11 @@ __global__ void __launch_bounds__(1024) k_closest_point(const Bvh4Node* __restri
 }
 
 // Four lanes per query (see bvh4_closest_quad): block of kQuadBlock lanes =
-// kQuadBlock / 4 queries; each lane keeps its own copy of the query's stack
-// in an LDS ring of kQuadCap entries.
+// kQuadBlock / 4 queries, each with an LDS ring of kQuadCap entries laid out
+// [slot][query], so the quads of a wave hit consecutive banks.
 constexpr int kQuadBlock = 256;
-constexpr int kQuadCap = 8;
+constexpr int kQuadQueries = kQuadBlock / 4;
+constexpr int kQuadCap = 32;  // 16 KB per block
 
 template <bool MASKED, bool COUNT>
 __global__ void __launch_bounds__(kQuadBlock) k_closest_point_quad(
@@ -61,7 +62,7 @@ __global__ void __launch_bounds__(kQuadBlock) k_closest_point_quad(
     const float* __restrict__ points, const uint8_t* __restrict__ active, int n, float* __restrict__ out_d2,
     float* __restrict__ out_cp, int* __restrict__ out_tri, int* __restrict__ out_overflow,
     uint32_t* __restrict__ out_nodes, uint32_t* __restrict__ out_tris) {
-  __shared__ uint2 s_ring[kQuadCap * kQuadBlock];
+  __shared__ uint2 s_ring[kQuadCap * kQuadQueries];
   const int t = static_cast<int>(threadIdx.x);
   const int k = t & 3;
   const int q = static_cast<int>((static_cast<int64_t>(blockIdx.x) * kQuadBlock + t) >> 2);
@@ -72,7 +73,7 @@ __global__ void __launch_bounds__(kQuadBlock) k_closest_point_quad(
     // inactive slot: neutral outputs (src/cuda/cubql_bvh.cu:198-204)
     h.d2 = 0.0f; h.cp = p; h.id = -1; h.overflow = 0; h.nodes = 0; h.tris = 0;
   } else if (finite3(p)) {
-    h = bvh4_closest_quad<kQuadCap, COUNT>(nodes, tris, p, delta, s_ring + t, kQuadBlock, k);
+    h = bvh4_closest_quad<kQuadCap, COUNT>(nodes, tris, p, delta, s_ring + (t >> 2), kQuadQueries, k);
   } else {
     h.d2 = FLT_MAX; h.cp = v3(0.0f, 0.0f, 0.0f); h.id = -1; h.overflow = 0; h.nodes = 0; h.tris = 0;
   }

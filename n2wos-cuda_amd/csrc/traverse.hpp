@@ -185,10 +185,12 @@ __device__ __forceinline__ Hit bvh4_closest(const Bvh4Node* __restrict__ nodes, 
 // Four lanes (one quad: lanes 4j .. 4j+3, which DPP quad permutations reach
 // in one instruction) share one query.  Lane k of the quad loads and tests
 // child k of each node and triangle k, k+4, ... of each leaf; every lane
-// holds the same copy of the query's state (stack, threshold, best), so the
-// quad's control flow is uniform and the visit order, the nodes and
-// triangle records visited and the result are exactly those of
-// bvh4_closest.  A wave serves 16 queries, so a far query keeps 4 lanes
+// holds the same copy of the query's state (threshold, best, stack
+// counters), so the quad's control flow is uniform and the visit order, the
+// nodes and triangle records visited and the result are exactly those of
+// bvh4_closest.  The LDS ring is one per quad (`sent` is the quad's column,
+// written by lane 0, read by all four); the scratch spill array is each
+// lane's own copy.  A wave serves 16 queries, so a far query keeps 4 lanes
 // busy instead of holding up 63 idle ones.
 #ifndef N2_EMULATED_WAVE
 template <int J>
@@ -273,7 +275,7 @@ __device__ __forceinline__ Hit bvh4_closest_quad(const Bvh4Node* __restrict__ no
     } else {                                           \
       ++cnt;                                           \
     }                                                  \
-    sent[top * stride] = make_uint2((R), __float_as_uint(K)); \
+    if (k == 0) sent[top * stride] = make_uint2((R), __float_as_uint(K)); \
     top = (top + 1) & (CAP - 1);                       \
   } while (0)
           if (nvalid >= 4) N2_PUSH(r3, k3);
