@@ -47,14 +47,16 @@ class CacheSetup:
 # 16 antithetic label pairs on every 8th (or 4th) point, 60 (100) Adam steps
 # at batch 2^15, and NC samples in antithetic pairs (the pair mean cancels the
 # odd part of the cache around each point, so few samples reach the cache's
-# bias floor).
+# bias floor).  The 2LMC run of the every-4th-point setup with 384 coupled
+# pairs per point reaches RMSE 1e-2 without bias, at about the cost of plain
+# WoS.
 DEFAULT_SETUPS = (
     CacheSetup("depth4_levels16", depth=4, train_steps=200, nc_wpp=(64, 256), lmc=((256, 16), (512, 16))),
     CacheSetup("depth1_levels2_anti", depth=1, cache_kw={"n_levels": 2}, train_pairs=16, label_stride=8,
                train_steps=60, batch=1 << 15, normalize_targets=True, antithetic=True, nc_wpp=(24, 32), lmc=()),
     CacheSetup("depth1_levels2_anti_more_labels", depth=1, cache_kw={"n_levels": 2}, train_pairs=16,
                label_stride=4, train_steps=100, batch=1 << 15, normalize_targets=True, antithetic=True,
-               nc_wpp=(32, 64), lmc=((64, 64),)),
+               nc_wpp=(32, 64), lmc=((64, 64), (64, 384))),
     CacheSetup("depth1_levels4", depth=1, cache_kw={"n_levels": 4}, train_steps=100, batch=1 << 16,
                nc_wpp=(96, 128, 192, 256), lmc=()),
 )
