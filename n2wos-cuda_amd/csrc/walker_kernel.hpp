@@ -34,9 +34,8 @@ __device__ __forceinline__ void cswap_key(float& ka, uint32_t& ra, float& kb, ui
 // visits a superset of the nodes the exact search needs, and its closest
 // points stay those of the brute-force scan.
 __device__ __forceinline__ float walker_prune_threshold(float best, float delta) {
-  if (best >= FLT_MAX) return FLT_MAX;
   const float t = hw_sqrt(best) * 1.0000019073486328125f + delta;
-  return t * t;
+  return best >= FLT_MAX ? FLT_MAX : t * t;
 }
 
 
@@ -229,12 +228,13 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
     const V3 q = closest_on_triangle_flat(p, ta, tb, tc);                          \
     const V3 e = q - p;                                                            \
     const float d2 = dot(e, e);                                                    \
-    if (d2 < bd2 || (d2 == bd2 && id < bid)) {                                     \
-      bd2 = d2;                                                                    \
-      bcp = q;                                                                     \
-      bid = id;                                                                    \
-      thr = fminf(thr, walker_prune_threshold(d2, a.delta));                       \
-    }                                                                              \
+    /* selects instead of a branch: the registers are updated in place */       \
+    const bool better_ = d2 < bd2 || (d2 == bd2 && id < bid);                      \
+    const float pt_ = fminf(thr, walker_prune_threshold(d2, a.delta));             \
+    bd2 = better_ ? d2 : bd2;                                                      \
+    bcp = v3(better_ ? q.x : bcp.x, better_ ? q.y : bcp.y, better_ ? q.z : bcp.z); \
+    bid = better_ ? id : bid;                                                      \
+    thr = better_ ? pt_ : thr;                                                     \
     ++ti;                                                                          \
     if (ti == te) N2_POP_NEXT();                                                   \
   } while (0)
