@@ -445,7 +445,7 @@ def main():
         extra["progressive_2lmc"] = PR.progressive_vs_serial(bvh, verts, pts, bc, truth, ttr_params, log=log)
         from n2wos_amd import serve as S
         extra["serving"] = S.serving_workload(eng, n_requests=args.serve_requests, rate_per_s=args.serve_rate,
-                                              config=S.ServerConfig(hbm_budget_bytes=1 << 30, tick_walks=1 << 22),
+                                              config=S.ServerConfig(hbm_budget_bytes=1 << 30),
                                               log=log)
         sv = extra["serving"]
         log(f"serving: p50 {sv['latency_p50_s']:.2f} s p99 {sv['latency_p99_s']:.2f} s, "
