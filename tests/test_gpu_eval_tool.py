@@ -59,3 +59,24 @@ def test_eval_tool_slice_outputs(tmp_path, torch_gpu):
         assert len(f.read().strip().splitlines()) == n + 1
     with open(pre + "_train_points.csv") as f:
         assert len(f.read().strip().splitlines()) == 4096 + 1
+
+
+def test_slice_mask_equals_reference_run(eng, torch_gpu):
+    """The slice sampler on the reference's own evaluation setting (bumpy
+    sphere 128 x 256, amplitude 0.15, normalised, xy slice at z = 0, 512 x 512,
+    mesh-bounds frame with 2% padding) gives the interior mask that the
+    reference's tool wrote for that run (figures/zebra_k8_m2_slice_mask.ppm),
+    bit for bit: frame, pixel positions and inside test all agree."""
+    from n2wos_amd import eval_nc as EV, geometry as G
+    g = np.load(os.path.join(ROOT, "tests", "golden", "slice_mask_zebra_k8_m2.npz"))
+    w, h = int(g["width"]), int(g["height"])
+    want = np.unpackbits(g["bits"])[: w * h].reshape(h, w).astype(bool)
+    opt = EV.parse_args(["--mesh", "procedural_bumpy_sphere", "--eval-mode", "slice", "--slice-view", "xy",
+                         "--slice-plane", "0.0", "--slice-width", str(w), "--slice-height", str(h)])
+    v, t, _, _, _ = EV.make_mesh(opt)
+    bvh = G.Bvh(v, t, leaf_size=opt.cubql_leaf_size, build_method=opt.cubql_build_method)
+    print("case: reference slice mask 512x512", flush=True)
+    pts, mask, _ = EV.slice_points(opt, bvh, v.min(0), v.max(0))
+    got = mask.reshape(h, w)[::-1].astype(bool)  # image rows: v descending
+    assert int(got.sum()) == int(g["inside_pixels"]) == len(pts)
+    assert np.array_equal(got, want)
