@@ -230,13 +230,24 @@ __device__ __forceinline__ h4_t mask_relu(f4_t d, h4_t h, float& dummy) {
   return r;
 }
 
+// The XCD (accelerator complex die) this wave runs on: hardware register
+// XCC_ID (20) on gfx94x / gfx950, 0-7 on MI355X.
+__device__ __forceinline__ uint32_t xcd_id() { return __builtin_amdgcn_s_getreg((31 << 11) | (0 << 6) | 20); }
+
+// Grid gradients go to copy (xcd_id & copy_mask) of grid_grad (copies
+// gstride floats apart), summed by Adam: small grids get one copy per XCD so
+// that fewer atomics meet on each address (copy_mask 7), large ones one copy
+// (copy_mask 0).  Every copy is updated with device-scope atomics, so the sum
+// is right whichever XCD a block runs on.
 __global__ void __launch_bounds__(kCacheBlock) k_cache_train_pass1(GridLayout gl, const float* __restrict__ grid,
                                                                   const _Float16* __restrict__ wh,
                                                                   const float* __restrict__ points,
                                                                   const float* __restrict__ targets, int n,
                                                                   float offset, float grad_scale, TrainStore st,
-                                                                  float* __restrict__ grid_grad,
-                                                                  float* __restrict__ loss_sum) {
+                                                                  float* __restrict__ grid_grad_copies,
+                                                                  float* __restrict__ loss_sum, uint32_t copy_mask,
+                                                                  size_t gstride) {
+  float* __restrict__ grid_grad = grid_grad_copies + (xcd_id() & copy_mask) * gstride;
   __shared__ _Float16 w[kCacheWeights];
   stage_weights(w, wh);
   const int lane = static_cast<int>(threadIdx.x & 63);
@@ -407,16 +418,22 @@ __global__ void __launch_bounds__(kCacheBlock) k_cache_wgrad(TrainStore st, int 
   }
 }
 
-// Adam (Kingma & Ba) on FP32 parameters with unscaled gradients; zeroes the
-// gradient.  Skips the update (and still zeroes) when *bad is set.
+// Adam (Kingma & Ba) on FP32 parameters with unscaled gradients, the sum of
+// `copies` gradient copies n floats apart (in copy order); zeroes them.
+// Skips the update (and still zeroes) when *bad is set.
 __global__ void k_adam(float* __restrict__ p, float* __restrict__ grad, float* __restrict__ m, float* __restrict__ v,
-                       size_t n, float lr_t, float beta1, float beta2, float eps, float inv_scale,
+                       size_t n, int copies, float lr_t, float beta1, float beta2, float eps, float inv_scale,
                        const int* __restrict__ bad) {
   const bool skip = *bad != 0;
   for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; k < n;
        k += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    const float gk = grad[k] * inv_scale;
+    float gs = grad[k];
     grad[k] = 0.0f;
+    for (int c = 1; c < copies; ++c) {
+      gs += grad[c * n + k];
+      grad[c * n + k] = 0.0f;
+    }
+    const float gk = gs * inv_scale;
     if (skip) continue;
     const float mk = beta1 * m[k] + (1.0f - beta1) * gk;
     const float vk = beta2 * v[k] + (1.0f - beta2) * gk * gk;
@@ -455,7 +472,8 @@ __global__ void k_init_uniform(float* __restrict__ p, size_t n, float lo, float 
 struct n2wos_cache {
   n2::GridLayout layout{};
   float* grid = nullptr;
-  float* grid_grad = nullptr;
+  float* grid_grad = nullptr;  // grad_copies copies of the grid's size
+  int grad_copies = 1;
   float* grid_m = nullptr;
   float* grid_v = nullptr;
   float* w = nullptr;
@@ -525,8 +543,12 @@ extern "C" int n2wos_cache_create(const n2wos_cache_config* cfg, n2wos_cache** o
       c->eps = cfg->epsilon;
       c->loss_scale = cfg->loss_scale;
       const size_t gf = grid_floats(c);
+      // one gradient copy per XCD for grids up to 2^20 floats (4 MB): measured
+      // on a 6-level 2^13 grid, a training step takes 0.24 instead of 0.33 ms;
+      // a 16-level 2^19 grid gains nothing and would read 8x the gradients
+      c->grad_copies = gf <= (size_t{1} << 20) ? 8 : 1;
       N2_HIP(hipMalloc(&c->grid, gf * 4));
-      N2_HIP(hipMalloc(&c->grid_grad, gf * 4));
+      N2_HIP(hipMalloc(&c->grid_grad, gf * 4 * c->grad_copies));
       N2_HIP(hipMalloc(&c->grid_m, gf * 4));
       N2_HIP(hipMalloc(&c->grid_v, gf * 4));
       N2_HIP(hipMalloc(&c->w, kCacheWeights * 4));
@@ -536,7 +558,7 @@ extern "C" int n2wos_cache_create(const n2wos_cache_config* cfg, n2wos_cache** o
       N2_HIP(hipMalloc(&c->wh, kCacheWeights * 2));
       N2_HIP(hipMalloc(&c->bad, 64));
       N2_HIP(hipMalloc(&c->loss, 64));
-      N2_HIP(hipMemset(c->grid_grad, 0, gf * 4));
+      N2_HIP(hipMemset(c->grid_grad, 0, gf * 4 * c->grad_copies));
       N2_HIP(hipMemset(c->grid_m, 0, gf * 4));
       N2_HIP(hipMemset(c->grid_v, 0, gf * 4));
       N2_HIP(hipMemset(c->w_grad, 0, kCacheWeights * 4));
@@ -619,13 +641,15 @@ extern "C" int n2wos_cache_train_step(n2wos_cache* c, const float* d_points, con
     const int tiles = batch / 16;
     const int blocks = std::max(1, std::min(2048, (tiles + 3) / 4));
     hipLaunchKernelGGL(k_cache_train_pass1, dim3(blocks), dim3(kCacheBlock), 0, s, c->layout, c->grid, c->wh,
-                       d_points, d_targets, n, c->out_offset, grad_scale, st, c->grid_grad, c->loss);
+                       d_points, d_targets, n, c->out_offset, grad_scale, st, c->grid_grad, c->loss,
+                       static_cast<uint32_t>(c->grad_copies - 1), grid_floats(c));
     const int chunk = 1024;
     const int waves = (batch + chunk - 1) / chunk;
     hipLaunchKernelGGL(k_cache_wgrad, dim3((waves + 3) / 4, 5), dim3(kCacheBlock), 0, s, st, chunk, c->w_grad);
     hipLaunchKernelGGL(k_check_finite, dim3(64), dim3(256), 0, s, c->w_grad, static_cast<size_t>(kCacheWeights),
                        c->bad);
-    hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, s, c->grid_grad, grid_floats(c), c->bad);
+    hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, s, c->grid_grad,
+                       grid_floats(c) * static_cast<size_t>(c->grad_copies), c->bad);
     c->step += 1;
     const double bc1 = 1.0 - std::pow(static_cast<double>(c->beta1), static_cast<double>(c->step));
     const double bc2 = 1.0 - std::pow(static_cast<double>(c->beta2), static_cast<double>(c->step));
@@ -633,9 +657,9 @@ extern "C" int n2wos_cache_train_step(n2wos_cache* c, const float* d_points, con
     const float eps_t = static_cast<float>(c->eps * std::sqrt(bc2));
     const float inv = 1.0f / c->loss_scale;
     hipLaunchKernelGGL(k_adam, dim3(64), dim3(256), 0, s, c->w, c->w_grad, c->w_m, c->w_v,
-                       static_cast<size_t>(kCacheWeights), lr_t, c->beta1, c->beta2, eps_t, inv, c->bad);
+                       static_cast<size_t>(kCacheWeights), 1, lr_t, c->beta1, c->beta2, eps_t, inv, c->bad);
     hipLaunchKernelGGL(k_adam, dim3(2048), dim3(256), 0, s, c->grid, c->grid_grad, c->grid_m, c->grid_v,
-                       grid_floats(c), lr_t, c->beta1, c->beta2, eps_t, inv, c->bad);
+                       grid_floats(c), c->grad_copies, lr_t, c->beta1, c->beta2, eps_t, inv, c->bad);
     hipLaunchKernelGGL(k_to_half, dim3((kCacheWeights + 255) / 256), dim3(256), 0, s, c->w, c->wh, kCacheWeights);
     N2_HIP(hipGetLastError());
     if (d_loss_out) N2_HIP(hipMemcpyAsync(d_loss_out, c->loss, 4, hipMemcpyDeviceToDevice, s));
@@ -744,14 +768,24 @@ extern "C" int n2wos_cache_gradients(n2wos_cache* c, const float* d_points, cons
     const int tiles = batch / 16;
     hipLaunchKernelGGL(k_cache_train_pass1, dim3(std::max(1, std::min(2048, (tiles + 3) / 4))), dim3(kCacheBlock), 0,
                        nullptr, c->layout, c->grid, c->wh, d_points, d_targets, n, c->out_offset,
-                       c->loss_scale / static_cast<float>(n), st, c->grid_grad, c->loss);
+                       c->loss_scale / static_cast<float>(n), st, c->grid_grad, c->loss,
+                       static_cast<uint32_t>(c->grad_copies - 1), grid_floats(c));
     const int waves = (batch + 1023) / 1024;
     hipLaunchKernelGGL(k_cache_wgrad, dim3((waves + 3) / 4, 5), dim3(kCacheBlock), 0, nullptr, st, 1024, c->w_grad);
     N2_HIP(hipGetLastError());
     N2_HIP(hipDeviceSynchronize());
     const size_t gf = grid_floats(c);
     std::vector<float> tmp;
-    if (grid_grad) N2_HIP(hipMemcpy(grid_grad, c->grid_grad, gf * 4, hipMemcpyDeviceToHost));
+    if (grid_grad) {
+      // the sum of the per-XCD copies, in copy order (as k_adam sums them)
+      tmp.resize(gf * static_cast<size_t>(c->grad_copies));
+      N2_HIP(hipMemcpy(tmp.data(), c->grid_grad, tmp.size() * 4, hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < gf; ++k) {
+        float s = tmp[k];
+        for (int x = 1; x < c->grad_copies; ++x) s += tmp[x * gf + k];
+        grid_grad[k] = s;
+      }
+    }
     if (weight_grad) N2_HIP(hipMemcpy(weight_grad, c->w_grad, kCacheWeights * 4, hipMemcpyDeviceToHost));
     const float inv = 1.0f / c->loss_scale;
     if (grid_grad)
@@ -762,7 +796,7 @@ extern "C" int n2wos_cache_gradients(n2wos_cache* c, const float* d_points, cons
       N2_HIP(hipMemcpy(loss, c->loss, 4, hipMemcpyDeviceToHost));
       *loss /= static_cast<float>(n);
     }
-    N2_HIP(hipMemset(c->grid_grad, 0, gf * 4));
+    N2_HIP(hipMemset(c->grid_grad, 0, gf * 4 * static_cast<size_t>(c->grad_copies)));
     N2_HIP(hipMemset(c->w_grad, 0, kCacheWeights * 4));
     return N2WOS_OK;
   });
