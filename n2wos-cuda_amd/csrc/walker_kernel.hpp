@@ -102,7 +102,11 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
   // arguments used only on rare paths (refill, walk step, walk end) are read
   // where they are used instead of being held in scalar registers across the
   // loop, which the node and triangle paths need
-#define KA N2_COLD_ARGS(a)
+#ifndef N2_EMULATED_WAVE
+#define KA (::n2::kernel_args<WosKernelArgs>())  // `a` is this kernel's only argument
+#else
+#define KA (&a)
+#endif
   const int lane = lane_id();
   const int wv = static_cast<int>(threadIdx.x) >> 6;
   uint2* sent = s_ent + threadIdx.x;

@@ -53,7 +53,6 @@ __device__ __forceinline__ kernarg_ptr<T> kernel_args() {
   asm volatile("" : "+s"(p));
   return p;
 }
-#define N2_COLD_ARGS(a) (::n2::kernel_args<WosKernelArgs>())
 #endif
 
 }  // namespace n2

@@ -96,7 +96,6 @@ namespace n2 {
 inline float hw_sqrt(float x) { return std::sqrt(x); }
 inline uint32_t opaque_u32(uint32_t v) { return v; }
 inline int opaque_lane_int(int v) { return v; }
-#define N2_COLD_ARGS(a) (&(a))
 inline int lane_id() { return emu::cur; }
 inline int rank_in(uint64_t mask) {
   const int l = emu::cur;
