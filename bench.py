@@ -83,10 +83,10 @@ def parse():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run a 1 GiB device copy (counter calibration) and one walker solve, then exit; "
                          "used under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
-    ap.add_argument("--section", choices=["walker", "cache", "closest_stress"], default=None,
+    ap.add_argument("--section", choices=["walker", "cache", "closest_stress", "poisson"], default=None,
                     help="run one section's GPU work only (one headline solve; the cache's training steps and "
-                         "inference; the closest-point stress sets), print a short JSON line and exit: the "
-                         "workload of the counter profiles")
+                         "inference; the closest-point stress sets; problem 3), print a short JSON line and "
+                         "exit: the workload of the counter profiles")
     return ap.parse_args()
 
 
@@ -243,10 +243,14 @@ def poisson_problem3(eng, torch, P, W) -> dict:
     tp = torch.from_numpy(pts).cuda()
     solver.solve(tp[:4096].contiguous(), 16, seed=1)
     torch.cuda.synchronize()
+    from n2wos_amd import hiptime
+    ev = hiptime.EventPair()
+    st = W.PointStats.zeros(len(pts))
     t0 = time.perf_counter()
-    st = solver.solve(tp, 512, seed=wp.seed)
+    solver.solve(tp, 512, stats=st, seed=wp.seed, events=ev.handles)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    kernel_ms = ev.elapsed()
     steps = int(st.steps.sum().item())
     truth = P.poisson_solution_f64(pts)
     mean = st.mean.cpu().numpy()
@@ -254,7 +258,7 @@ def poisson_problem3(eng, torch, P, W) -> dict:
     err = mean - truth
     z = err / np.sqrt(np.maximum(var / 512, 1e-300))
     out = {"triangles": int(len(t)), "points": 262144, "walks_per_point": 512, "seconds": dt,
-           "walk_steps_per_s": steps / dt, "mean_steps_per_walk": steps / (262144 * 512),
+           "walk_steps_per_s": steps / dt, "walker_kernel_ms": kernel_ms, "mean_steps_per_walk": steps / (262144 * 512),
            "rmse_vs_fp64": float(np.sqrt(np.mean(err ** 2))), "mean_error": float(err.mean()),
            "frac_within_3_sigma": float(np.mean(np.abs(z[var > 0]) <= 3.0)), "host_mesh_and_bvh_s": build_s,
            "forced_walks": int(st.forced.sum().item())}
@@ -317,6 +321,9 @@ def main():
         return
     if args.section == "closest_stress":
         print(json.dumps({"section": "closest_stress", **closest_point_stress(eng, torch, P)}))
+        return
+    if args.section == "poisson":
+        print(json.dumps({"section": "poisson", **poisson_problem3(eng, torch, P, W)}))
         return
 
     if args.traffic_probe:
