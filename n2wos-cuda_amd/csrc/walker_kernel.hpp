@@ -27,6 +27,17 @@ __device__ __forceinline__ void cswap_key(float& ka, uint32_t& ra, float& kb, ui
   rb = rhi;
 }
 
+// Squared distance from p to a box, with the sum of squares fused (two FMAs):
+// a pruning key only, so a rounding different from box_dist2's by an ulp
+// stays far inside the threshold's margin (see below).
+__device__ __forceinline__ float box_dist2_fused(V3 p, float lox, float loy, float loz, float hix, float hiy,
+                                                 float hiz) {
+  const float dx = fmaxf(fmaxf(lox - p.x, p.x - hix), 0.0f);
+  const float dy = fmaxf(fmaxf(loy - p.y, p.y - hiy), 0.0f);
+  const float dz = fmaxf(fmaxf(loz - p.z, p.z - hiz), 0.0f);
+  return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
 // The walker's pruning threshold: prune_threshold with the hardware square
 // root and the relative margin raised from 2^-20 to 2^-19.  A square root at
 // most 2 ulp low still gives sqrt(best)(1 + 2^-19)(1 - 2^-22) > sqrt(best)(1 +
@@ -194,10 +205,10 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
 #define N2_NODE_VISIT(lox, loy, loz, hix, hiy, hiz, chf)                          \
   do {                                                                             \
     if (COUNT) ++n_nodes;                                                          \
-    float k0 = box_dist2(p, lox.x, loy.x, loz.x, hix.x, hiy.x, hiz.x);             \
-    float k1 = box_dist2(p, lox.y, loy.y, loz.y, hix.y, hiy.y, hiz.y);             \
-    float k2 = box_dist2(p, lox.z, loy.z, loz.z, hix.z, hiy.z, hiz.z);             \
-    float k3 = box_dist2(p, lox.w, loy.w, loz.w, hix.w, hiy.w, hiz.w);             \
+    float k0 = box_dist2_fused(p, lox.x, loy.x, loz.x, hix.x, hiy.x, hiz.x);       \
+    float k1 = box_dist2_fused(p, lox.y, loy.y, loz.y, hix.y, hiy.y, hiz.y);       \
+    float k2 = box_dist2_fused(p, lox.z, loy.z, loz.z, hix.z, hiy.z, hiz.z);       \
+    float k3 = box_dist2_fused(p, lox.w, loy.w, loz.w, hix.w, hiy.w, hiz.w);       \
     const int nvalid = (k0 <= thr) + (k1 <= thr) + (k2 <= thr) + (k3 <= thr);      \
     if (nvalid == 0) {                                                             \
       N2_POP_NEXT();                                                               \
