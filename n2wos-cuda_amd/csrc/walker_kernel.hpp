@@ -101,7 +101,9 @@ constexpr int kUpdate = 3;  // query finished: take the walk step (or end the wa
 // node visits, triangle tests and walk steps is exactly the sequential one.
 template <int CAP, int S, bool POISSON, bool COUNT>
 __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persistent(WosKernelArgs a) {
-  __shared__ uint2 s_ent[CAP * kBlock];
+  // CAP ring slots per lane plus one spare slot that takes the writes of
+  // absent entries, so a node visit pushes without branches
+  __shared__ uint2 s_ent[(CAP + 1) * kBlock];
   __shared__ float s_val[kWaves][S][64];
   __shared__ int s_unit[kWaves][S];
   __shared__ int s_rem[kWaves][S];
@@ -161,9 +163,10 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
 #define N2_POP_NEXT()                                \
   do {                                               \
     bool pop_found_ = false;                         \
-    while (cnt > 0 || scnt > 0) {                    \
+    uint32_t pop_ref_ = 0;                           \
+    /* the loop carries only the stack state; the entry found is entered after it */ \
+    while (!pop_found_ && (cnt > 0 || scnt > 0)) {   \
       float pop_key_;                                \
-      uint32_t pop_ref_;                             \
       if (cnt > 0) {                                 \
         top = (top - 1) & (CAP - 1);                 \
         --cnt;                                       \
@@ -176,29 +179,40 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
         pop_key_ = __uint_as_float(static_cast<uint32_t>(se_ >> 32)); \
         pop_ref_ = static_cast<uint32_t>(se_);       \
       }                                              \
-      if (pop_key_ <= thr) {                         \
-        N2_ENTER(pop_ref_);                          \
-        pop_found_ = true;                           \
-        break;                                       \
-      }                                              \
+      pop_found_ = pop_key_ <= thr;                  \
     }                                                \
-    if (!pop_found_) phase = kUpdate;                \
-  } while (0)
-#define N2_PUSH(R, K)                                \
-  do {                                               \
-    if (cnt == CAP) {                                \
-      if (scnt < kSpill) {                           \
-        const uint2 ve_ = sent[top * kBlock];        \
-        spill_e[scnt] = (static_cast<uint64_t>(ve_.y) << 32) | ve_.x; \
-        ++scnt;                                      \
-      } else {                                       \
-        ovf = 1;                                     \
-      }                                              \
+    if (pop_found_) {                                \
+      N2_ENTER(pop_ref_);                            \
     } else {                                         \
-      ++cnt;                                         \
+      phase = kUpdate;                               \
     }                                                \
-    sent[top * kBlock] = make_uint2((R), __float_as_uint(K)); \
-    top = (top + 1) & (CAP - 1);                     \
+  } while (0)
+  // push the last M (0-3) of (R3, K3), (R2, K2), (R1, K1) in that order, as
+  // M single pushes would: R1 ends on top.  When the ring overflows, its
+  // oldest entries move to the spill stack first (oldest first), or are
+  // dropped with the overflow flag set when that is full too.
+#define N2_PUSH3(M, R1, K1, R2, K2, R3, K3)                                       \
+  do {                                                                            \
+    const int m_ = (M);                                                           \
+    if (cnt + m_ > CAP) {                                                         \
+      for (int j_ = CAP - cnt; j_ < m_; ++j_) {                                   \
+        if (scnt < kSpill) {                                                      \
+          const uint2 ve_ = sent[((top + j_) & (CAP - 1)) * kBlock];              \
+          spill_e[scnt] = (static_cast<uint64_t>(ve_.y) << 32) | ve_.x;           \
+          ++scnt;                                                                 \
+        } else {                                                                  \
+          ovf = 1;                                                                \
+        }                                                                         \
+      }                                                                           \
+    }                                                                             \
+    const int s1_ = m_ >= 1 ? ((top + m_ - 1) & (CAP - 1)) : CAP;                 \
+    const int s2_ = m_ >= 2 ? ((top + m_ - 2) & (CAP - 1)) : CAP;                 \
+    const int s3_ = m_ >= 3 ? top : CAP;                                          \
+    sent[s3_ * kBlock] = make_uint2((R3), __float_as_uint(K3));                   \
+    sent[s2_ * kBlock] = make_uint2((R2), __float_as_uint(K2));                   \
+    sent[s1_ * kBlock] = make_uint2((R1), __float_as_uint(K1));                   \
+    top = (top + m_) & (CAP - 1);                                                 \
+    cnt = cnt + m_ < CAP ? cnt + m_ : CAP;                                        \
   } while (0)
 
   // one node visit: the node's line is (lox, loy, loz, hix, hiy, hiz, child refs)
@@ -226,9 +240,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
       cswap_key(k0, r0, k2, r2);                                                   \
       cswap_key(k1, r1, k3, r3);                                                   \
       cswap_key(k1, r1, k2, r2);                                                   \
-      if (nvalid >= 4) N2_PUSH(r3, k3);                                            \
-      if (nvalid >= 3) N2_PUSH(r2, k2);                                            \
-      if (nvalid >= 2) N2_PUSH(r1, k1);                                            \
+      N2_PUSH3(nvalid - 1, r1, k1, r2, k2, r3, k3);                                \
       N2_ENTER(r0);                                                                \
     }                                                                              \
   } while (0)
@@ -501,7 +513,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
 #undef KA
 #undef N2_ENTER
 #undef N2_POP_NEXT
-#undef N2_PUSH
+#undef N2_PUSH3
 #undef N2_NODE_VISIT
 #undef N2_TRI_TEST
   if (COUNT) {
