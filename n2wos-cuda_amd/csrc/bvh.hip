@@ -50,11 +50,12 @@ __global__ void __launch_bounds__(1024) k_closest_point(const Bvh4Node* __restri
 }
 
 // Four lanes per query (see bvh4_closest_quad): block of kQuadBlock lanes =
-// kQuadBlock / 4 queries, each with an LDS ring of kQuadCap entries laid out
-// [slot][query], so the quads of a wave hit consecutive banks.
+// kQuadBlock / 4 queries, each with an LDS ring of kQuadCap entries and one
+// spare slot laid out [slot][query], so the quads of a wave hit consecutive
+// banks.
 constexpr int kQuadBlock = 256;
 constexpr int kQuadQueries = kQuadBlock / 4;
-constexpr int kQuadCap = 32;  // 16 KB per block
+constexpr int kQuadCap = 32;  // 16.5 KB per block with the spare slot
 
 template <bool MASKED, bool COUNT>
 __global__ void __launch_bounds__(kQuadBlock) k_closest_point_quad(
@@ -62,7 +63,7 @@ __global__ void __launch_bounds__(kQuadBlock) k_closest_point_quad(
     const float* __restrict__ points, const uint8_t* __restrict__ active, int n, float* __restrict__ out_d2,
     float* __restrict__ out_cp, int* __restrict__ out_tri, int* __restrict__ out_overflow,
     uint32_t* __restrict__ out_nodes, uint32_t* __restrict__ out_tris) {
-  __shared__ uint2 s_ring[kQuadCap * kQuadQueries];
+  __shared__ uint2 s_ring[(kQuadCap + 1) * kQuadQueries];
   const int t = static_cast<int>(threadIdx.x);
   const int k = t & 3;
   const int q = static_cast<int>((static_cast<int64_t>(blockIdx.x) * kQuadBlock + t) >> 2);

@@ -188,8 +188,9 @@ __device__ __forceinline__ Hit bvh4_closest(const Bvh4Node* __restrict__ nodes, 
 // holds the same copy of the query's state (threshold, best, stack
 // counters), so the quad's control flow is uniform and the visit order, the
 // nodes and triangle records visited and the result are exactly those of
-// bvh4_closest.  The LDS ring is one per quad (`sent` is the quad's column,
-// written by lane 0, read by all four); the scratch spill array is each
+// bvh4_closest.  The LDS ring is one per quad (`sent` is the quad's column
+// of CAP + 1 slots, the last a spare that takes the writes of absent
+// entries; written by lane 0, read by all four); the scratch spill array is each
 // lane's own copy.  A wave serves 16 queries, so a far query keeps 4 lanes
 // busy instead of holding up 63 idle ones.
 #ifndef N2_EMULATED_WAVE
@@ -262,26 +263,32 @@ __device__ __forceinline__ Hit bvh4_closest_quad(const Bvh4Node* __restrict__ no
           cswap(k0, r0, s0, k2, r2, s2);
           cswap(k1, r1, s1, k3, r3, s3);
           cswap(k1, r1, s1, k2, r2, s2);
-#define N2_PUSH(R, K)                                  \
-  do {                                                 \
-    if (cnt == CAP) {                                  \
-      if (scnt < kSpill) {                             \
-        const uint2 ve_ = sent[top * stride];          \
-        spill_e[scnt] = (static_cast<uint64_t>(ve_.y) << 32) | ve_.x; \
-        ++scnt;                                        \
-      } else {                                         \
-        h.overflow = 1;                                \
-      }                                                \
-    } else {                                           \
-      ++cnt;                                           \
-    }                                                  \
-    if (k == 0) sent[top * stride] = make_uint2((R), __float_as_uint(K)); \
-    top = (top + 1) & (CAP - 1);                       \
-  } while (0)
-          if (nvalid >= 4) N2_PUSH(r3, k3);
-          if (nvalid >= 3) N2_PUSH(r2, k2);
-          if (nvalid >= 2) N2_PUSH(r1, k1);
-#undef N2_PUSH
+          // push r3, r2, r1 (the last nvalid - 1 of them, farthest first) in
+          // one step: the oldest ring entries move to the spill stack first
+          // when the ring overflows; absent entries are written to the spare
+          // slot CAP, so lane 0 writes without branches
+          const int m = nvalid - 1;
+          if (cnt + m > CAP) {
+            for (int j = CAP - cnt; j < m; ++j) {
+              if (scnt < kSpill) {
+                const uint2 ve = sent[((top + j) & (CAP - 1)) * stride];
+                spill_e[scnt] = (static_cast<uint64_t>(ve.y) << 32) | ve.x;
+                ++scnt;
+              } else {
+                h.overflow = 1;
+              }
+            }
+          }
+          if (k == 0) {
+            const int q1 = m >= 1 ? ((top + m - 1) & (CAP - 1)) : CAP;
+            const int q2 = m >= 2 ? ((top + m - 2) & (CAP - 1)) : CAP;
+            const int q3 = m >= 3 ? top : CAP;
+            sent[q3 * stride] = make_uint2(r3, __float_as_uint(k3));
+            sent[q2 * stride] = make_uint2(r2, __float_as_uint(k2));
+            sent[q1 * stride] = make_uint2(r1, __float_as_uint(k1));
+          }
+          top = (top + m) & (CAP - 1);
+          cnt = cnt + m < CAP ? cnt + m : CAP;
           cur = r0;
           continue;
         }
