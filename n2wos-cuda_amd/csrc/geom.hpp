@@ -101,9 +101,20 @@ N2_HD V3 closest_on_triangle_flat(V3 p, V3 a, V3 b, V3 c) {
   const bool rAC = !done4 && vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f;
   const bool rBC = !done4 && !rAC && va <= 0.0f && e43 >= 0.0f && e56 >= 0.0f;
   const bool rFace = !done4 && !rAC && !rBC;
-  // the region's division
-  const float num = rAB ? d1 : (rAC ? d2 : (rBC ? e43 : 1.0f));
-  const float den = rAB ? (d1 - d3) : (rAC ? (d2 - d6) : (rBC ? e43 + e56 : (va + vb + vc)));
+  // the region's division.  Every candidate denominator is computed and one
+  // is selected (the regions exclude each other), so the compiler emits
+  // selects instead of branches around each candidate.
+  const float den_ab = d1 - d3;
+  const float den_ac = d2 - d6;
+  const float den_bc = e43 + e56;
+  const float den_face = va + vb + vc;
+  float num = 1.0f, den = den_face;
+  num = rBC ? e43 : num;
+  den = rBC ? den_bc : den;
+  num = rAC ? d2 : num;
+  den = rAC ? den_ac : den;
+  num = rAB ? d1 : num;
+  den = rAB ? den_ab : den;
   const float q = num / den;
   // edge: base + dir * q; face: (a + ab * (vb q)) + ac * (vc q).  Selects
   // are written per component so they stay register selects.
