@@ -256,8 +256,11 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
     const V3 e = q - p;                                                            \
     const float d2 = dot(e, e);                                                    \
     /* selects instead of a branch: the registers are updated in place */       \
-    const bool better_ = d2 < bd2 || (d2 == bd2 && id < bid);                      \
-    const float pt_ = fminf(thr, walker_prune_threshold(d2, a.delta));             \
+    /* bitwise, not short-circuit: no branch.  A NaN d2 is never better, and */ \
+    /* the threshold is never NaN, so the select is the minimum */              \
+    const bool better_ = (d2 < bd2) | ((d2 == bd2) & (id < bid));                  \
+    const float ptc_ = walker_prune_threshold(d2, a.delta);                        \
+    const float pt_ = ptc_ < thr ? ptc_ : thr;                                     \
     bd2 = better_ ? d2 : bd2;                                                      \
     bcp = v3(better_ ? q.x : bcp.x, better_ ? q.y : bcp.y, better_ ? q.z : bcp.z); \
     bid = better_ ? id : bid;                                                      \
