@@ -256,8 +256,10 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
     const V3 e = q - p;                                                            \
     const float d2 = dot(e, e);                                                    \
     /* selects instead of a branch: the registers are updated in place */       \
-    /* bitwise, not short-circuit: no branch.  A NaN d2 is never better, and */ \
-    /* the threshold is never NaN, so the select is the minimum */              \
+    /* bitwise, not short-circuit: no branch.  pt_ is used only when d2 is  */ \
+    /* better, so d2 is not NaN; the threshold can be NaN only for a NaN     */ \
+    /* point (vertices are finite), whose d2 is NaN: the select is fminf's   */ \
+    /* result wherever it is used */                                            \
     const bool better_ = (d2 < bd2) | ((d2 == bd2) & (id < bid));                  \
     const float ptc_ = walker_prune_threshold(d2, a.delta);                        \
     const float pt_ = ptc_ < thr ? ptc_ : thr;                                     \
