@@ -139,8 +139,10 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
   float src = 0.0f;
   float thr0 = FLT_MAX;
   // query state
-  uint32_t cur = 0;
-  int ti = 0, te = 0, top = 0, cnt = 0, scnt = 0, bid = -1, ovf = 0;
+  // one position register: the node index in the node phase, the next
+  // triangle record in the triangle phase (a lane is in one at a time)
+  int cur = 0;
+  int te = 0, top = 0, cnt = 0, scnt = 0, bid = -1, ovf = 0;
   float thr = FLT_MAX, bd2 = FLT_MAX;
   V3 bcp = v3(0.0f, 0.0f, 0.0f);
   uint64_t spill_e[kSpill];
@@ -151,11 +153,11 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
   do {                                                    \
     const uint32_t enter_ref_ = (r);                      \
     if (enter_ref_ & kLeafFlag) {                         \
-      ti = static_cast<int>((enter_ref_ >> 5) & 0x3ffffffu); \
-      te = ti + static_cast<int>(enter_ref_ & 31u);       \
+      cur = static_cast<int>((enter_ref_ >> 5) & 0x3ffffffu); \
+      te = cur + static_cast<int>(enter_ref_ & 31u);      \
       phase = kTri;                                       \
     } else {                                              \
-      cur = enter_ref_;                                   \
+      cur = static_cast<int>(enter_ref_);                 \
       phase = kNode;                                      \
     }                                                     \
   } while (0)
@@ -267,8 +269,8 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
     bcp = v3(better_ ? q.x : bcp.x, better_ ? q.y : bcp.y, better_ ? q.z : bcp.z); \
     bid = better_ ? id : bid;                                                      \
     thr = better_ ? pt_ : thr;                                                     \
-    ++ti;                                                                          \
-    if (ti == te) N2_POP_NEXT();                                                   \
+    ++cur;                                                                         \
+    if (cur == te) N2_POP_NEXT();                                                   \
   } while (0)
 
   while (true) {
@@ -509,7 +511,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
     } else {
       // ------------------------------------------------------ one triangle
       if (phase == kTri) {
-        const float4* tp = reinterpret_cast<const float4*>(a.tris + ti);
+        const float4* tp = reinterpret_cast<const float4*>(a.tris + cur);
         const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
         N2_TRI_TEST(t0, t1, t2);
       }

@@ -75,8 +75,8 @@ inline double __shfl_down(double v, int dlt) {
   if (lane == 0) ++emu::iters;                                                                     \
   wave_sync();                                                                                     \
   if (emu::iters > emu::iter_limit) {                                                              \
-    std::fprintf(stderr, "lane %d phase %d step %d point %d walk %d cur %u ti %d te %d cnt %d "   \
-                 "scnt %d bid %d bd2 %g thr %g thr0 %g\n", lane, phase, step, point, walk, cur, ti, te, \
+    std::fprintf(stderr, "lane %d phase %d step %d point %d walk %d cur %d te %d cnt %d "          \
+                 "scnt %d bid %d bd2 %g thr %g thr0 %g\n", lane, phase, step, point, walk, cur, te,    \
                  cnt, scnt, bid, bd2, thr, thr0);                                                  \
     if (lane == 0) emu::hung = 1;                                                                  \
     wave_sync();                                                                                   \
