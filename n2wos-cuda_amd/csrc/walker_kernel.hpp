@@ -12,6 +12,7 @@ namespace n2 {
 #ifndef N2_EMU_LOOP_HOOK
 #define N2_EMU_LOOP_HOOK()
 #define N2_EMU_TRACE(sN, sT, sU)
+#define N2_EMU_TRI_HOOK(idx)
 #endif
 
 // Compare-exchange by key for the walker's child sort, branch-free.  Ties
@@ -270,7 +271,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
     bid = better_ ? id : bid;                                                      \
     thr = better_ ? pt_ : thr;                                                     \
     ++cur;                                                                         \
-    if (cur == te) N2_POP_NEXT();                                                   \
+    if (cur >= te) N2_POP_NEXT();                                                   \
   } while (0)
 
   while (true) {
@@ -455,8 +456,10 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
             // against d + R): bound the next query by it.  The margin covers
             // the FP32 rounding of the closest point and of the distance.
             const V3 eo = p - bcp;
+            // clamped to FLT_MAX: a bound that overflows is the unbounded
+            // query, and an empty child slot (box key +inf) stays unvisited
             const float tb = hw_sqrt(dot(eo, eo)) * 1.0000152587890625f + 4.0f * a.delta;
-            thr0 = tb * tb;
+            thr0 = fminf(tb * tb, FLT_MAX);
             bd2 = FLT_MAX;
             bcp = v3(0.0f, 0.0f, 0.0f);
             bid = -1;
@@ -511,6 +514,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
     } else {
       // ------------------------------------------------------ one triangle
       if (phase == kTri) {
+        N2_EMU_TRI_HOOK(cur);
         const float4* tp = reinterpret_cast<const float4*>(a.tris + cur);
         const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
         N2_TRI_TEST(t0, t1, t2);

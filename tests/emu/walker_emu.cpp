@@ -40,6 +40,8 @@ int hung = 0;
 int diverged = 0;
 long path_iters[3];
 long path_lanes[3];
+long tri_count = -1;  // records in the triangle array (-1: not checked)
+long tri_oob = 0;     // triangle-record loads outside [0, tri_count)
 
 inline void arrive(int op) {
   lanes[cur].op = op;
@@ -81,6 +83,15 @@ inline double __shfl_down(double v, int dlt) {
     if (lane == 0) emu::hung = 1;                                                                  \
     wave_sync();                                                                                   \
     break;                                                                                         \
+  }
+// Bounds check of every triangle-record load against the array's size.  A
+// load outside it is counted, then served from record 0 with the leaf ended
+// there, so the failure is reported instead of reading past the array.
+#define N2_EMU_TRI_HOOK(idx)                                                                       \
+  if (emu::tri_count >= 0 && ((idx) < 0 || (idx) >= emu::tri_count)) {                             \
+    ++emu::tri_oob;                                                                                \
+    cur = 0;                                                                                       \
+    te = 1;                                                                                        \
   }
 #define N2_EMU_TRACE(sN, sT, sU)                                                                   \
   if (lane == 0) {                                                                                 \
@@ -219,6 +230,13 @@ extern "C" int emu_walker(const void* nodes, const void* tris, float delta, cons
 }
 
 extern "C" long emu_iterations() { return emu::iters; }
+// Sets the triangle array's size for the load bounds check (-1: off) and
+// clears the count of loads outside it.
+extern "C" void emu_set_tri_count(long n) {
+  emu::tri_count = n;
+  emu::tri_oob = 0;
+}
+extern "C" long emu_tri_oob() { return emu::tri_oob; }
 extern "C" void emu_set_trace(long n) { emu::trace = n; }
 extern "C" void emu_path_stats(long* iters, long* lanes) {
   for (int k = 0; k < 3; ++k) {

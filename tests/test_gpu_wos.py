@@ -168,3 +168,31 @@ def test_partial_grid_equals_full_grid(eng, torch_gpu, m1):
         assert np.array_equal(a.mean.cpu().numpy().view(np.uint64), b.mean.cpu().numpy().view(np.uint64))
         assert np.array_equal(a.m2.cpu().numpy().view(np.uint64), b.m2.cpu().numpy().view(np.uint64))
         assert np.array_equal(a.steps.cpu().numpy(), b.steps.cpu().numpy())
+
+
+@pytest.mark.parametrize("mesh", ["sphere8x16", "mesh1"])
+def test_exterior_start_points_bit_exact(eng, orc, torch_gpu, mesh):
+    """Walks from points outside the closed mesh drift away until the step cap
+    (256); past about 1.8e19 their step bound no longer fits in FP32 and the
+    query runs unbounded.  Values and step counts equal the oracle's walks, and
+    no walk reports an overflow."""
+    from n2wos_amd import problems as P, wos as W
+    torch = torch_gpu
+    if mesh == "mesh1":
+        v, t = P.make_mesh("mesh1")
+    else:
+        v0, t = eng.uv_sphere(8, 16, eng.SPHERE_SIN6, [0.1])
+        v, _, _ = eng.normalize_to_unit_radius(v0)
+    bvh = eng.Bvh(v, t)
+    pts = np.array([[1.6, 0.0, 0.0], [0.0, -1.7, 0.3], [2.5, 2.5, -2.5], [0.3, 0.2, 0.1]], np.float32)
+    bc = P.charges_boundary(0)
+    wp = W.WalkParams(max_steps=256, epsilon=1e-3, seed=77)
+    print("case: exterior start points %s 4x64" % mesh, flush=True)
+    st = run(torch, bvh, bc, wp, pts, 64)
+    gv = st.extra["walk_values"].cpu().numpy()
+    gs = st.extra["walk_steps"].cpu().numpy()
+    ov, os_, _, _ = orc.Bvh2(v, t).walks(pts, 64, max_steps=256, eps=1e-3, step_scale=0.999, bmode=bc.mode,
+                                         bparams=bc.oracle_params(), seed=77, nthreads=16)
+    assert np.array_equal(gs, os_)
+    assert np.array_equal(bits(gv), bits(ov))
+    assert (gs[:3 * 64] == 256).mean() > 0.5  # most exterior walks run to the cap
