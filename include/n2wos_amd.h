@@ -233,6 +233,26 @@ int n2wos_wos_solve(const n2wos_bvh* bvh, const n2wos_boundary* boundary, const 
                     void* d_workspace, int64_t workspace_bytes, float* d_walk_values, int* d_walk_steps,
                     void* stream);
 
+/* Largest walker workspace (n2wos_wos_workspace_size, bytes) that
+ * n2wos_wos_solve_noalloc serves: 32 MiB.  For example 170,132 points x 256
+ * walks need 20.4 MB; 680,000 points x 256 walks do not fit. */
+#define N2WOS_NOALLOC_SCRATCH_BYTES (32LL << 20)
+/* n2wos_wos_solve without the workspace arguments, for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  No counterpart in the reference.  The
+ * walker's workspace is a file-scope device array of
+ * N2WOS_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until
+ * the process ends; a job whose workspace would exceed it returns
+ * N2WOS_ERR_INVALID before anything is enqueued.  The call allocates nothing
+ * and gives n2wos_wos_solve's results bit for bit.  Every call on one GPU
+ * shares that GPU's array (it keeps the last call's contents): calls that
+ * overlap on one GPU, from two streams or two threads, race, so the caller
+ * serialises them. */
+int n2wos_wos_solve_noalloc(const n2wos_bvh* bvh, const n2wos_boundary* boundary, const n2wos_walk_params* params,
+                            const float* d_points, int n_points, int walks_per_point, uint32_t point_offset,
+                            double* d_n, double* d_mean, double* d_m2, uint64_t* d_steps, uint32_t* d_forced,
+                            int accumulate, float* d_walk_values, int* d_walk_steps, void* stream);
+
 /* ------------------------------------------------ neural-cache WoS and 2LMC */
 /* Walks of the neural-cache estimators (replacing launch_nc_hybrid_prefix and
  * launch_nc_2lmc_prefix_continue, include/n2wos/tcnn_nc_wos.hpp:79-134):

@@ -37,6 +37,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 #include "api_internal.hpp"
 #include "walker_kernel.hpp"
@@ -274,57 +276,132 @@ extern "C" int64_t n2wos_wos_workspace_size(int n_points, int walks_per_point) {
   return wos_workspace_bytes(n_points, walks_per_point);
 }
 
+namespace {
+
+// The body of both plain-solve entry points; d_workspace holds at least
+// wos_workspace_bytes(n_points, walks_per_point) bytes.
+int wos_solve_impl(const n2wos_bvh* bvh, const n2wos_boundary* boundary, const n2wos_walk_params* wp,
+                   const float* d_points, int n_points, int walks_per_point, uint32_t point_offset, double* d_n,
+                   double* d_mean, double* d_m2, uint64_t* d_steps, uint32_t* d_forced, int accumulate,
+                   void* d_workspace, float* d_walk_values, int* d_walk_steps, void* stream) {
+  check_walk_params(wp);
+  const Boundary bc = to_boundary(boundary);
+  WosRun r{};
+  r.d_points = d_points;
+  r.n_points = n_points;
+  r.walks_per_point = walks_per_point;
+  r.point_offset = point_offset;
+  r.max_steps = wp->max_steps;
+  r.epsilon = wp->epsilon;
+  r.step_scale = wp->step_scale;
+  r.seed = wp->seed;
+  r.rng_stream = wp->rng_stream;
+  r.source = wp->source;
+  r.max_blocks = wp->max_blocks;
+  r.d_n = d_n;
+  r.d_mean = d_mean;
+  r.d_m2 = d_m2;
+  r.d_steps = d_steps;
+  r.d_forced = d_forced;
+  r.accumulate = accumulate;
+  r.d_workspace = d_workspace;
+  r.d_walk_values = d_walk_values;
+  r.d_walk_steps = d_walk_steps;
+  r.d_visits = static_cast<uint64_t*>(wp->d_visit_counts);
+  r.mode = kModePlain;
+  r.depth = 0;
+  r.ev_begin = wp->event_begin;
+  r.ev_end = wp->event_end;
+  run_wos(bvh, bc, r, static_cast<hipStream_t>(stream));
+  return N2WOS_OK;
+}
+
+// Argument checks shared by both plain-solve entry points (the workspace is
+// checked by each).
+void check_solve_args(const n2wos_bvh* bvh, const n2wos_boundary* boundary, const n2wos_walk_params* wp,
+                      const float* d_points, int n_points, int walks_per_point, uint32_t point_offset, double* d_n,
+                      double* d_mean, double* d_m2, uint64_t* d_steps, uint32_t* d_forced, float* d_walk_values,
+                      int* d_walk_steps) {
+  N2_REQUIRE(bvh && boundary && wp, N2WOS_ERR_NULL, "NULL handle, boundary or parameters");
+  N2_REQUIRE(bvh->d_nodes && bvh->d_tris, N2WOS_ERR_EMPTY, "solve called on an empty BVH");
+  N2_REQUIRE(n_points >= 0, N2WOS_ERR_INVALID, "n_points must be non-negative");
+  N2_REQUIRE(walks_per_point >= 1, N2WOS_ERR_INVALID, "walks_per_point must be >= 1");
+  if (n_points == 0) return;
+  N2_REQUIRE(static_cast<int64_t>(n_points) * ((walks_per_point + 63) / 64) < (int64_t{1} << 31),
+             N2WOS_ERR_INVALID, "too many work units for one launch");
+  N2_REQUIRE(static_cast<int64_t>(n_points) + point_offset <= (int64_t{1} << 32), N2WOS_ERR_INVALID,
+             "point ids exceed 32 bits");
+  N2_REQUIRE(d_points && d_n && d_mean && d_m2 && d_steps && d_forced, N2WOS_ERR_NULL, "NULL device pointer");
+  N2_REQUIRE((d_walk_values == nullptr) == (d_walk_steps == nullptr), N2WOS_ERR_NULL,
+             "per-walk outputs must be both set or both NULL");
+}
+
+}  // namespace
+
 extern "C" int n2wos_wos_solve(const n2wos_bvh* bvh, const n2wos_boundary* boundary, const n2wos_walk_params* wp,
                                const float* d_points, int n_points, int walks_per_point, uint32_t point_offset,
                                double* d_n, double* d_mean, double* d_m2, uint64_t* d_steps, uint32_t* d_forced,
                                int accumulate, void* d_workspace, int64_t workspace_bytes, float* d_walk_values,
                                int* d_walk_steps, void* stream) {
   return guarded([&]() -> int {
-    N2_REQUIRE(bvh && boundary && wp, N2WOS_ERR_NULL, "NULL handle, boundary or parameters");
-    N2_REQUIRE(bvh->d_nodes && bvh->d_tris, N2WOS_ERR_EMPTY, "solve called on an empty BVH");
-    N2_REQUIRE(n_points >= 0, N2WOS_ERR_INVALID, "n_points must be non-negative");
-    N2_REQUIRE(walks_per_point >= 1, N2WOS_ERR_INVALID, "walks_per_point must be >= 1");
+    check_solve_args(bvh, boundary, wp, d_points, n_points, walks_per_point, point_offset, d_n, d_mean, d_m2,
+                     d_steps, d_forced, d_walk_values, d_walk_steps);
     if (n_points == 0) return N2WOS_OK;
-    N2_REQUIRE(static_cast<int64_t>(n_points) * ((walks_per_point + 63) / 64) < (int64_t{1} << 31),
-               N2WOS_ERR_INVALID, "too many work units for one launch");
-    N2_REQUIRE(static_cast<int64_t>(n_points) + point_offset <= (int64_t{1} << 32), N2WOS_ERR_INVALID,
-               "point ids exceed 32 bits");
-    N2_REQUIRE(d_points && d_n && d_mean && d_m2 && d_steps && d_forced && d_workspace, N2WOS_ERR_NULL,
-               "NULL device pointer");
-    N2_REQUIRE((d_walk_values == nullptr) == (d_walk_steps == nullptr), N2WOS_ERR_NULL,
-               "per-walk outputs must be both set or both NULL");
+    N2_REQUIRE(d_workspace, N2WOS_ERR_NULL, "NULL device pointer");
     N2_REQUIRE(workspace_bytes >= wos_workspace_bytes(n_points, walks_per_point), N2WOS_ERR_INVALID,
                "workspace too small (n2wos_wos_workspace_size)");
-    check_walk_params(wp);
-    const Boundary bc = to_boundary(boundary);
-    WosRun r{};
-    r.d_points = d_points;
-    r.n_points = n_points;
-    r.walks_per_point = walks_per_point;
-    r.point_offset = point_offset;
-    r.max_steps = wp->max_steps;
-    r.epsilon = wp->epsilon;
-    r.step_scale = wp->step_scale;
-    r.seed = wp->seed;
-    r.rng_stream = wp->rng_stream;
-    r.source = wp->source;
-    r.max_blocks = wp->max_blocks;
-    r.d_n = d_n;
-    r.d_mean = d_mean;
-    r.d_m2 = d_m2;
-    r.d_steps = d_steps;
-    r.d_forced = d_forced;
-    r.accumulate = accumulate;
-    r.d_workspace = d_workspace;
-    r.d_walk_values = d_walk_values;
-    r.d_walk_steps = d_walk_steps;
-    r.d_visits = static_cast<uint64_t*>(wp->d_visit_counts);
-    r.mode = kModePlain;
-    r.depth = 0;
-    r.ev_begin = wp->event_begin;
-    r.ev_end = wp->event_end;
-    run_wos(bvh, bc, r, static_cast<hipStream_t>(stream));
-    return N2WOS_OK;
+    return wos_solve_impl(bvh, boundary, wp, d_points, n_points, walks_per_point, point_offset, d_n, d_mean, d_m2,
+                          d_steps, d_forced, accumulate, d_workspace, d_walk_values, d_walk_steps, stream);
+  });
+}
+
+// ------------------------------------------------------------ no allocation
+// The walker workspace of n2wos_wos_solve_noalloc: one file-scope device
+// array, which every GPU the library runs on holds until the process ends.
+// In host code `scratch` names a host-side stand-in, not device memory: its
+// device address on the current GPU comes from hipGetSymbolAddress, taken
+// once per device on that device's first call and kept in scratch_ptr.
+static __device__ __align__(256) unsigned char scratch[N2WOS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of this GPU's scratch array (the current device's).
+void* scratch_ptr() {
+  int dev = 0;
+  N2_HIP(hipGetDevice(&dev));
+  N2_REQUIRE(dev >= 0 && dev < kMaxScratchDevices, N2WOS_ERR_UNSUPPORTED, "device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    N2_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    N2_REQUIRE(p != nullptr, N2WOS_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" int n2wos_wos_solve_noalloc(const n2wos_bvh* bvh, const n2wos_boundary* boundary,
+                                       const n2wos_walk_params* wp, const float* d_points, int n_points,
+                                       int walks_per_point, uint32_t point_offset, double* d_n, double* d_mean,
+                                       double* d_m2, uint64_t* d_steps, uint32_t* d_forced, int accumulate,
+                                       float* d_walk_values, int* d_walk_steps, void* stream) {
+  return guarded([&]() -> int {
+    check_solve_args(bvh, boundary, wp, d_points, n_points, walks_per_point, point_offset, d_n, d_mean, d_m2,
+                     d_steps, d_forced, d_walk_values, d_walk_steps);
+    if (n_points == 0) return N2WOS_OK;
+    const int64_t need = wos_workspace_bytes(n_points, walks_per_point);
+    N2_REQUIRE(need <= N2WOS_NOALLOC_SCRATCH_BYTES, N2WOS_ERR_INVALID,
+               "job needs " + std::to_string(need) + " bytes of walker workspace; n2wos_wos_solve_noalloc holds " +
+                   std::to_string(static_cast<long long>(N2WOS_NOALLOC_SCRATCH_BYTES)) +
+                   " (split the points into smaller calls, or use n2wos_wos_solve with a workspace)");
+    // the unit counter is cleared by a kernel in run_wos; every other part
+    // of the workspace is written before it is read
+    return wos_solve_impl(bvh, boundary, wp, d_points, n_points, walks_per_point, point_offset, d_n, d_mean, d_m2,
+                          d_steps, d_forced, accumulate, scratch_ptr(), d_walk_values, d_walk_steps, stream);
   });
 }
 
