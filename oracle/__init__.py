@@ -94,6 +94,8 @@ def lib():
         L.orc_round_half.argtypes = [C.c_double]
         L.orc_round_half.restype = C.c_double
         L.orc_cache_forward_fp16.argtypes = L.orc_cache_forward.argtypes
+        L.orc_cache_encode.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _f32p, _f32p, _f32p, _f32p,
+                                       C.c_long, C.c_int, _f64p]
         L.orc_cache_grads_fp16.argtypes = L.orc_cache_grads.argtypes + [C.c_double]
         L.orc_cache_grads_fp16.restype = C.c_double
         _lib = L
@@ -293,6 +295,15 @@ def cache_forward(cfg, grid, weights, pts, fp16=False):
     out = np.zeros(len(pts))
     fn = lib().orc_cache_forward_fp16 if fp16 else lib().orc_cache_forward
     fn(*_cfg_args(cfg), _f32(grid), _f32(weights), pts, len(pts), out)
+    return out
+
+
+def cache_encode(cfg, grid, pts, f32=False):
+    """Hash-grid features (n, 32) at (n, 3) points: column 2 l + f is feature
+    f of level l.  f32=True maps and weights in FP32 like the product kernels."""
+    pts = _f32(pts).reshape(-1, 3)
+    out = np.zeros((len(pts), 32))
+    lib().orc_cache_encode(*_cfg_args(cfg), _f32(grid), pts, len(pts), int(f32), out)
     return out
 
 

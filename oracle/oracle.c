@@ -1283,6 +1283,48 @@ static double orc_mlp_fwd(const orc_grid* g, const float* grid, const float* wt,
   return orc_mlp_fwd_ex(g, grid, wt, p, x, h, 0);
 }
 
+/* The hash-grid encoding alone: out[32 * s + 2 l + f] = feature f of level l
+ * at point s (levels >= n_levels are 0).  f32 != 0 uses the FP32 unit-cube
+ * mapping and corner weights of the product kernels, else FP64. */
+int orc_cache_encode(int n_levels, int log2_t, int base_res, double growth, const float* in_min,
+                     const float* in_extent, const float* grid, const float* pts, long n, int f32, double* out) {
+  orc_grid g;
+  orc_grid_init(&g, n_levels, log2_t, base_res, growth, in_min, in_extent);
+  for (long s = 0; s < n; ++s) {
+    double* x = out + 32 * s;
+    for (int k = 0; k < 32; ++k) x[k] = 0.0;
+    if (f32) {
+      float u[3];
+      orc_unit_f32(&g, pts + 3 * s, u);
+      for (int l = 0; l < g.n_levels; ++l) {
+        uint32_t idx[8];
+        float w[8];
+        orc_corners_f32(&g, l, u, idx, w);
+        float f0 = 0.0f, f1 = 0.0f;
+        for (int c = 0; c < 8; ++c) {
+          f0 += w[c] * grid[2 * (size_t)idx[c]];
+          f1 += w[c] * grid[2 * (size_t)idx[c] + 1];
+        }
+        x[2 * l] = f0;
+        x[2 * l + 1] = f1;
+      }
+    } else {
+      double u[3];
+      orc_unit(&g, pts + 3 * s, u);
+      for (int l = 0; l < g.n_levels; ++l) {
+        uint32_t idx[8];
+        double w[8];
+        orc_corners(&g, l, u, idx, w);
+        for (int c = 0; c < 8; ++c) {
+          x[2 * l] += w[c] * (double)grid[2 * (size_t)idx[c]];
+          x[2 * l + 1] += w[c] * (double)grid[2 * (size_t)idx[c] + 1];
+        }
+      }
+    }
+  }
+  return 0;
+}
+
 int orc_cache_forward(int n_levels, int log2_t, int base_res, double growth, const float* in_min,
                       const float* in_extent, const float* grid, const float* wt, const float* pts, long n,
                       double* out) {

@@ -279,3 +279,101 @@ def test_half_rounding_matches_numpy_float16(orc):
     want = vals.astype(half).astype(np.float64)
     assert np.array_equal(np.signbit(got[want == 0]), np.signbit(want[want == 0]))
     assert np.array_equal(got, want)
+
+
+def _sphere_angles(stacks, slices):
+    """(theta, phi) of every UV-sphere vertex in the generator's order, FP64:
+    north pole, rings i = 1..stacks-1 of slices vertices, south pole."""
+    i = np.repeat(np.arange(1, stacks), slices)
+    j = np.tile(np.arange(slices), stacks - 1)
+    th = np.concatenate([[0.0], np.pi * i / stacks, [np.pi]])
+    ph = np.concatenate([[0.0], 2 * np.pi * j / slices, [0.0]])
+    return th, ph
+
+
+@pytest.mark.parametrize("kind,stacks,slices", [(1, 64, 128), (1, 128, 256), (2, 512, 1024)])
+def test_radius_formulas_at_vertices(orc, eng, kind, stacks, slices):
+    """Before normalisation every vertex lies at the benchmark radius
+    r(theta, phi) evaluated in FP64: 1 + 0.1 sin6t sin6p (meshes 1, 2) and
+    1 + 0.06 sin5t sin7p + 0.03 sin17t cos13p (mesh 3).  The bound covers the
+    FP32 rounding of the angles and of the position; a dropped term, a wrong
+    frequency or a swapped angle is off by 1e-2 or more."""
+    th, ph = _sphere_angles(stacks, slices)
+    if kind == 1:
+        want = 1.0 + 0.1 * np.sin(6 * th) * np.sin(6 * ph)
+    else:
+        want = 1.0 + 0.06 * np.sin(5 * th) * np.sin(7 * ph) + 0.03 * np.sin(17 * th) * np.cos(13 * ph)
+    vo, to = orc.uv_sphere(stacks, slices, kind, [0.1])
+    ve, te = eng.uv_sphere(stacks, slices, kind, [0.1])
+    assert np.array_equal(bits(vo), bits(ve)) and np.array_equal(to, te)
+    r = np.linalg.norm(vo.astype(np.float64), axis=1)
+    assert np.max(np.abs(r - want)) < 2e-5
+    # the direction is (sin t cos p, sin t sin p, cos t)
+    dirs = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], axis=1)
+    assert np.max(np.abs(vo.astype(np.float64) - want[:, None] * dirs)) < 2e-5
+
+
+def _unit_cfg(**kw):
+    from n2wos_amd.cache import CacheConfig
+    return CacheConfig(origin=(0.0, 0.0, 0.0), extent=(1.0, 1.0, 1.0), **kw)
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_hash_grid_dense_level_reproduces_linear_field(orc, eng, f32):
+    """Trilinear interpolation reproduces a linear function exactly: with the
+    dense level's entries set to a + b x/N + c y/N + d z/N at vertex (x, y, z)
+    (index x + y (N+1) + z (N+1)^2), the feature at any point u of the unit
+    cube is a + b u_x + c u_y + d u_z.  A swapped weight (f for 1 - f), a
+    transposed index or a wrong stride breaks it."""
+    cfg = _unit_cfg(n_levels=2, log2_table_size=19, base_resolution=16, growth=1.5)
+    lay = orc.cache_layout(2, 19, 16, 1.5)
+    grid = np.zeros(2 * lay["total_entries"], np.float32)
+    coef = {0: (0.25, 0.5, -0.75, 1.25), 1: (-0.5, 0.125, 0.375, -0.625)}
+    for lvl in range(2):
+        assert lay["dense"][lvl]
+        n = lay["res"][lvl]
+        s = n + 1
+        idx = np.arange(s ** 3)
+        x, y, z = idx % s, (idx // s) % s, idx // (s * s)
+        for f in range(2):
+            a, b, c, d = coef[f] if lvl == 0 else coef[1 - f]
+            grid[2 * (lay["offsets"][lvl] + idx) + f] = a + b * x / n + c * y / n + d * z / n
+    pts = np.random.default_rng(5).uniform(0, 1, (3000, 3)).astype(np.float32)
+    enc = orc.cache_encode(cfg, grid, pts, f32=f32)
+    u = pts.astype(np.float64)
+    tol = 2e-6 if not f32 else 2e-5
+    for lvl in range(2):
+        for f in range(2):
+            a, b, c, d = coef[f] if lvl == 0 else coef[1 - f]
+            assert np.max(np.abs(enc[:, 2 * lvl + f] - (a + b * u[:, 0] + c * u[:, 1] + d * u[:, 2]))) < tol
+    assert np.all(enc[:, 4:] == 0.0)
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_hash_grid_vertex_values_and_spatial_hash(orc, eng, f32):
+    """At a grid vertex the feature equals that vertex's table entry.  On a
+    hashed level (16 levels from N_min 16, growth 1.5, T = 2^19: level 4 has
+    N = 81, (N+1)^3 > T) the entry of vertex (x, y, z) is
+    (x xor 2654435761 y xor 805459861 z) mod T, checked at hand-picked integer
+    coordinates; on a dense level it is x + y (N+1) + z (N+1)^2."""
+    cfg = _unit_cfg(n_levels=5, log2_table_size=19, base_resolution=16, growth=1.5)
+    lay = orc.cache_layout(5, 19, 16, 1.5)
+    assert lay["res"][4] == 81 and not lay["dense"][4] and lay["dense"][3]
+    T = 1 << 19
+    grid = np.random.default_rng(6).uniform(-1, 1, 2 * lay["total_entries"]).astype(np.float32)
+    verts = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (3, 7, 11), (11, 7, 3), (80, 2, 45), (45, 2, 80),
+             (17, 64, 33), (81, 81, 81)]
+    for lvl in (3, 4):
+        n = lay["res"][lvl]
+        pts = np.array([[x / n, y / n, z / n] for x, y, z in verts], np.float32)
+        enc = orc.cache_encode(cfg, grid, pts, f32=f32)
+        for k, (x, y, z) in enumerate(verts):
+            if lay["dense"][lvl]:
+                x, y, z = (min(c, n) for c in (x, y, z))
+                local = x + y * (n + 1) + z * (n + 1) ** 2
+            else:
+                m32 = 0xFFFFFFFF
+                local = (x ^ ((y * 2654435761) & m32) ^ ((z * 805459861) & m32)) & (T - 1)
+            e = lay["offsets"][lvl] + local
+            for f in range(2):
+                assert abs(enc[k, 2 * lvl + f] - grid[2 * e + f]) < (2e-6 if not f32 else 2e-5), (lvl, x, y, z, f)
