@@ -253,6 +253,45 @@ int n2wos_wos_solve_noalloc(const n2wos_bvh* bvh, const n2wos_boundary* boundary
                             double* d_n, double* d_mean, double* d_m2, uint64_t* d_steps, uint32_t* d_forced,
                             int accumulate, float* d_walk_values, int* d_walk_steps, void* stream);
 
+/* ------------------------------------------------ along-walk control variate */
+/* A field C for the control variate: values at the (res + 1)^3 vertices of
+ * a box grid, vertex (x, y, z) at d_values[x + (res+1) (y + (res+1) z)],
+ * vertex position origin + (x, y, z) / cell_inverse; read by trilinear
+ * interpolation, coordinates clamped to the box.  points: 2 (antipodal pair)
+ * or 6 (octahedron turned by a uniform angle about the step direction);
+ * beta scales the correction (1 = full). */
+typedef struct {
+  const float* d_values;
+  int res;
+  int points;
+  float origin[3];
+  float cell_inverse[3];
+  float beta;
+} n2wos_cv_field;
+
+/* Plain walks with the along-walk control variate: each walk's value is
+ *   W - beta sum_k [C(X_k + R_k d_k) - (1/m) sum_j C(X_k + R_k s_j)],
+ * where X_k + R_k d_k is the walk's next position and s_j the m design
+ * directions around d_k, each uniform on the sphere: every term has zero
+ * mean, so the estimate is unbiased for any field, and a field close to the
+ * solution (the learned cache evaluated on the grid) cancels most of the
+ * walk's variance.  field NULL or points 0: plain walks.  Statistics,
+ * workspace, point_offset and per-walk outputs as n2wos_wos_solve.  With
+ * path_len > 0 the walker also records, per walk (walk index i = point *
+ * walks_per_point + walk), its first path_len positions X_k (d_path_pos[3
+ * (i path_len + k)]) and q_k = S_k - beta (terms of steps < k)
+ * (d_path_q[i path_len + k]), S_k the source sum before step k: entries k <=
+ * steps of walk i are written, and d_walk_values[i] - q_k is an unbiased
+ * estimate of the solution at X_k (training labels for the cache).  No
+ * counterpart in the reference; its two-level estimator
+ * (src/tools/eval_tcnn_nc_wos.cu:767-783) uses the cache at one depth only. */
+int n2wos_cv_solve(const n2wos_bvh* bvh, const n2wos_boundary* boundary, const n2wos_walk_params* params,
+                   const float* d_points, int n_points, int walks_per_point, uint32_t point_offset,
+                   const n2wos_cv_field* field, double* d_n, double* d_mean, double* d_m2, uint64_t* d_steps,
+                   uint32_t* d_forced, int accumulate, void* d_workspace, int64_t workspace_bytes,
+                   float* d_walk_values, int* d_walk_steps, float* d_path_pos, float* d_path_q, int path_len,
+                   void* stream);
+
 /* ------------------------------------------------ neural-cache WoS and 2LMC */
 /* Walks of the neural-cache estimators (replacing launch_nc_hybrid_prefix and
  * launch_nc_2lmc_prefix_continue, include/n2wos/tcnn_nc_wos.hpp:79-134):

@@ -92,6 +92,8 @@ _SIGNATURES = {
     "n2wos_nc_walks": ([vp, vp, vp, vp, i32, i32, C.c_uint32, i32, i32, vp, vp, vp, vp, vp, vp, i64, vp], i32),
     "n2wos_nc_combine": ([i32, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp, vp], i32),
     "n2wos_wos_solve": ([vp, vp, vp, vp, i32, i32, C.c_uint32, vp, vp, vp, vp, vp, i32, vp, i64, vp, vp, vp], i32),
+    "n2wos_cv_solve": ([vp, vp, vp, vp, i32, i32, C.c_uint32, vp, vp, vp, vp, vp, vp, i32, vp, i64, vp, vp, vp, vp,
+                        i32, vp], i32),
     "n2wos_wos_solve_noalloc": ([vp, vp, vp, vp, i32, i32, C.c_uint32, vp, vp, vp, vp, vp, i32, vp, vp, vp], i32),
 }
 

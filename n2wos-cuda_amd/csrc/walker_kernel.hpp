@@ -3,6 +3,7 @@
 // the CPU tests, which supplies its own definitions of the wave intrinsics).
 #pragma once
 
+#include "cvfield.hpp"
 #include "traverse.hpp"
 #include "wave.hpp"
 #include "wos.hpp"
@@ -100,7 +101,11 @@ constexpr int kUpdate = 3;  // query finished: take the walk step (or end the wa
 // for paths they do not take, and a lane whose query ends starts its next
 // query without waiting for the rest of the wave.  Each lane's own sequence of
 // node visits, triangle tests and walk steps is exactly the sequential one.
-template <int CAP, int S, bool POISSON, bool COUNT>
+//
+// EXT selects the extended walker (plain walks only): the along-walk control
+// variate over a field (cvfield.hpp) and the optional path record.  The plain
+// instantiation carries none of it.
+template <int CAP, int S, bool POISSON, bool COUNT, bool EXT = false>
 __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persistent(WosKernelArgs a) {
   // CAP ring slots per lane plus one spare slot that takes the writes of
   // absent entries, so a node visit pushes without branches
@@ -138,6 +143,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
   uint32_t ctr_point = 0, ctr_walk = 0;  // Philox counter words of the walk
   V3 p = v3(0.0f, 0.0f, 0.0f);
   float src = 0.0f;
+  float cvs = 0.0f;  // extended walker: sum of the control-variate terms
   float thr0 = FLT_MAX;
   // query state
   // one position register: the node index in the node phase, the next
@@ -297,9 +303,17 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
             no_more = true;
             break;
           }
-          const int pt = u / KA->units_per_point;
-          const int chunk = u - pt * KA->units_per_point;
-          const int ucnt = min(64, KA->walks_per_point - 64 * chunk);
+          int pt, chunk, ucnt;
+          if (EXT && KA->pack_shift >= 0) {
+            // packed unit: 64 / wpp whole points
+            pt = u << (6 - KA->pack_shift);
+            chunk = 0;
+            ucnt = min(64, (KA->n_points - pt) << KA->pack_shift);
+          } else {
+            pt = u / KA->units_per_point;
+            chunk = u - pt * KA->units_per_point;
+            ucnt = min(64, KA->walks_per_point - 64 * chunk);
+          }
           if (lane == 0) {
             s_unit[wv][fs] = u;
             s_rem[wv][fs] = ucnt;
@@ -321,8 +335,13 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
           // every walk of a point shares its first query: computed once per point
           slot = fill_slot;
           wiu = fill_next + r;
-          point = fill_point;
-          walk = fill_w0 + wiu;
+          if (EXT && KA->pack_shift >= 0) {
+            point = fill_point + (wiu >> KA->pack_shift);
+            walk = wiu & ((1 << KA->pack_shift) - 1);
+          } else {
+            point = fill_point;
+            walk = fill_w0 + wiu;
+          }
           if (KA->point_rng) {
             ctr_point = KA->point_rng[2 * point];
             ctr_walk = KA->point_rng[2 * point + 1] + static_cast<uint32_t>(walk);
@@ -332,6 +351,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
           }
           step = 0;
           src = 0.0f;
+          if (EXT) cvs = 0.0f;
           thr0 = FLT_MAX;
           p = v3(KA->points[3 * point], KA->points[3 * point + 1], KA->points[3 * point + 2]);
           bd2 = KA->first_d2[point];
@@ -375,6 +395,17 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
           // prefix mode ends a walk at depth m (after the boundary test, as the
           // reference's hybrid prefix does, src/cuda/cubql_bvh.cu:1486-1510)
           const bool stop_prefix = !at_boundary && KA->mode == kModePrefix && step >= KA->depth;
+          if (EXT) {
+            // path record: X_step and q_step, the terminal position included
+            if (step < KA->path_len) {
+              const int64_t pidx =
+                  (static_cast<int64_t>(point) * KA->walks_per_point + walk) * KA->path_len + step;
+              KA->path_pos[3 * pidx] = p.x;
+              KA->path_pos[3 * pidx + 1] = p.y;
+              KA->path_pos[3 * pidx + 2] = p.z;
+              KA->path_q[pidx] = src - KA->cv_beta * cvs;
+            }
+          }
           if (at_boundary || stop_prefix) {
             float gval = 0.0f;
             if (at_boundary) {
@@ -390,7 +421,8 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
                 gval = boundary_value(b0, bcp);
               }
             }
-            const float value = at_boundary ? src + gval : src;
+            float value = at_boundary ? src + gval : src;
+            if (EXT) value = value - KA->cv_beta * cvs;
             if (KA->mode != kModePlain) {
               const int64_t sidx = static_cast<int64_t>(point) * KA->walks_per_point + walk;
               if (KA->mode == kModeCoupled) KA->nc_full[sidx] = value;
@@ -404,9 +436,16 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
               }
             }
             s_val[wv][slot][wiu] = value;
-            atomicAdd(&s_steps[wv][slot], static_cast<uint32_t>(step));
-            if (forced) atomicAdd(&s_forced[wv][slot], 1u);
-            if (ovf) atomicAdd(&s_ovf[wv][slot], 1u);
+            if (EXT && KA->pack_shift >= 0) {
+              // packed unit: the point's own counters (integer sums, any order)
+              atomicAdd(&KA->unit_steps[point], static_cast<uint32_t>(step));
+              if (forced || ovf)
+                atomicAdd(&KA->unit_flags[point], (forced ? 1u : 0u) + (ovf ? 0x10000u : 0u));
+            } else {
+              atomicAdd(&s_steps[wv][slot], static_cast<uint32_t>(step));
+              if (forced) atomicAdd(&s_forced[wv][slot], 1u);
+              if (ovf) atomicAdd(&s_ovf[wv][slot], 1u);
+            }
             if (KA->walk_values) {
               const int64_t sidx = static_cast<int64_t>(point) * KA->walks_per_point + walk;
               KA->walk_values[sidx] = value;
@@ -439,15 +478,26 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
             V3 dir = sphere_dir(rr.x, rr.y);
             dir = v3(sgn * dir.x, sgn * dir.y, sgn * dir.z);
             const float R = KA->step_scale * d;
+            // the control variate's frame angle: word 2 of the direction draw
+            // (Laplace) or word 3 of the source draw (Poisson), both unused
+            uint32_t psi_word = rr.z;
             if (POISSON) {
               const U4 t = philox4x32_10(U4{static_cast<uint32_t>(step), cw, ctr_point, 2u * KA->stream + 1u},
                                          key_lo, key_hi);
+              psi_word = t.w;
               const float rho = median3(u01(rr.z), u01(rr.w), u01(t.x));
               V3 e = sphere_dir(t.y, t.z);
               e = v3(sgn * e.x, sgn * e.y, sgn * e.z);
               const float Rr = R * rho;
               const V3 y = v3(p.x + Rr * e.x, p.y + Rr * e.y, p.z + Rr * e.z);
               src = src + (R * R) * poisson_source(y) / 6.0f;
+            }
+            if (!EXT) (void)psi_word;
+            if (EXT) {
+              if (KA->cv_points > 0)
+                cvs = cvs + cv_term(KA->cv_values, KA->cv_res, KA->cv_origin[0], KA->cv_origin[1],
+                                    KA->cv_origin[2], KA->cv_inv_h[0], KA->cv_inv_h[1], KA->cv_inv_h[2],
+                                    KA->cv_points, p, R, dir, psi_word);
             }
             p = v3(p.x + R * dir.x, p.y + R * dir.y, p.z + R * dir.z);
             ++step;
@@ -483,19 +533,38 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
         done &= done - 1;
         const int s = __shfl(slot, l);
         const int u = s_unit[wv][s];
-        const int pt = u / KA->units_per_point;
-        const int ucnt = min(64, KA->walks_per_point - 64 * (u - pt * KA->units_per_point));
+        const bool packed = EXT && KA->pack_shift >= 0;
+        int pt, ucnt, span;
+        if (packed) {
+          pt = u << (6 - KA->pack_shift);
+          ucnt = min(64, (KA->n_points - pt) << KA->pack_shift);
+          span = 1 << KA->pack_shift;  // one point's walks: the tree stops at its segment
+        } else {
+          pt = u / KA->units_per_point;
+          ucnt = min(64, KA->walks_per_point - 64 * (u - pt * KA->units_per_point));
+          span = 64;
+        }
         double n = lane < ucnt ? 1.0 : 0.0;
         double m = lane < ucnt ? static_cast<double>(s_val[wv][s][lane]) : 0.0;
         double q = 0.0;
+        // Chan tree over lane pairs at strides 1, 2, 4, ...: within a packed
+        // unit's segments it gives each point exactly what the full tree
+        // gives a one-point unit (merges with empty lanes change nothing)
 #pragma unroll
         for (int stride = 1; stride < 64; stride *= 2) {
           const double nb = __shfl_down(n, stride);
           const double mb = __shfl_down(m, stride);
           const double qb = __shfl_down(q, stride);
-          if ((lane & (2 * stride - 1)) == 0) chan_merge(n, m, q, nb, mb, qb, n, m, q);
+          if (stride < span && (lane & (2 * stride - 1)) == 0) chan_merge(n, m, q, nb, mb, qb, n, m, q);
         }
-        if (lane == 0) {
+        if (packed) {
+          if ((lane & (span - 1)) == 0 && lane < ucnt) {
+            const int pp = pt + (lane >> KA->pack_shift);
+            KA->unit_mean[pp] = m;
+            KA->unit_m2[pp] = q;
+          }
+          if (lane == 0) s_unit[wv][s] = -1;
+        } else if (lane == 0) {
           KA->unit_mean[u] = m;
           KA->unit_m2[u] = q;
           KA->unit_steps[u] = s_steps[wv][s];

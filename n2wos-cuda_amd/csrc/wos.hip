@@ -37,6 +37,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <string>
 
@@ -80,32 +81,49 @@ __global__ void k_point_merge(int n_points, int units_per_point, int walks_per_p
 
 __global__ void k_zero_counter(int* c) { *c = 0; }
 
+__global__ void k_zero_u32(uint32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// Packed units of the extended walker: walks_per_point a power of two <= 32
+// (log2 of it, else -1).
+int pack_shift_for(int walks_per_point) {
+  for (int k = 0; k <= 5; ++k)
+    if (walks_per_point == (1 << k)) return k;
+  return -1;
+}
+
 template <int CAP, int S, bool POISSON>
-void launch_wos(const WosKernelArgs& a, int grid, hipStream_t stream) {
-  if (a.visits)
+void launch_wos(const WosKernelArgs& a, int grid, hipStream_t stream, bool ext) {
+  if (ext)
+    hipLaunchKernelGGL((k_wos_persistent<CAP, S, POISSON, false, true>), dim3(grid), dim3(kBlock), 0, stream, a);
+  else if (a.visits)
     hipLaunchKernelGGL((k_wos_persistent<CAP, S, POISSON, true>), dim3(grid), dim3(kBlock), 0, stream, a);
   else
     hipLaunchKernelGGL((k_wos_persistent<CAP, S, POISSON, false>), dim3(grid), dim3(kBlock), 0, stream, a);
 }
 
-template <int CAP, int S, bool POISSON>
+template <int CAP, int S, bool POISSON, bool EXT>
 int max_blocks_per_cu() {
   int b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wos_persistent<CAP, S, POISSON, false>, kBlock, 0) !=
-      hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wos_persistent<CAP, S, POISSON, false, EXT>, kBlock,
+                                                   0) != hipSuccess)
     return 1;
   return std::max(1, b);
 }
 
 // Kernel variant: LDS ring capacity ("ring": 16 or 8 entries per lane).
 template <int S, bool POISSON>
-int variant_blocks_per_cu(int cap) {
-  return cap == 16 ? max_blocks_per_cu<16, S, POISSON>() : max_blocks_per_cu<8, S, POISSON>();
+int variant_blocks_per_cu(int cap, bool ext) {
+  if (ext) return cap == 16 ? max_blocks_per_cu<16, S, POISSON, true>() : max_blocks_per_cu<8, S, POISSON, true>();
+  return cap == 16 ? max_blocks_per_cu<16, S, POISSON, false>() : max_blocks_per_cu<8, S, POISSON, false>();
 }
 
 template <int S, bool POISSON>
-void launch_variant(const WosKernelArgs& a, int grid, hipStream_t stream, int cap) {
-  if (cap == 16) launch_wos<16, S, POISSON>(a, grid, stream); else launch_wos<8, S, POISSON>(a, grid, stream);
+void launch_variant(const WosKernelArgs& a, int grid, hipStream_t stream, int cap, bool ext) {
+  if (cap == 16) launch_wos<16, S, POISSON>(a, grid, stream, ext);
+  else launch_wos<8, S, POISSON>(a, grid, stream, ext);
 }
 
 }  // namespace
@@ -134,14 +152,15 @@ void check_walk_params(const n2wos_walk_params* wp) {
              "unknown source term");
 }
 
-int wos_full_grid(int source) {
+int wos_full_grid(int source, bool ext) {
   int dev = 0, cus = 0;
   N2_HIP(hipGetDevice(&dev));
   N2_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   constexpr int kSlots = 8;
   const bool poisson = source == N2WOS_SOURCE_POISSON_SINES;
   const int cap = tuning_value("ring", 8) == 16 ? 16 : 8;
-  const int per_cu = poisson ? variant_blocks_per_cu<kSlots, true>(cap) : variant_blocks_per_cu<kSlots, false>(cap);
+  const int per_cu =
+      poisson ? variant_blocks_per_cu<kSlots, true>(cap, ext) : variant_blocks_per_cu<kSlots, false>(cap, ext);
   return cus * per_cu;
 }
 
@@ -163,6 +182,12 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
   a.walks_per_point = r.walks_per_point;
   a.units_per_point = (r.walks_per_point + 63) / 64;
   a.total_units = r.n_points * a.units_per_point;
+  a.n_points = r.n_points;
+  a.pack_shift = r.ext ? pack_shift_for(r.walks_per_point) : -1;
+  if (a.pack_shift >= 0) {
+    const int ppu = 64 >> a.pack_shift;
+    a.total_units = (r.n_points + ppu - 1) / ppu;
+  }
   a.max_steps = r.max_steps;
   a.epsilon = r.epsilon;
   a.step_scale = r.step_scale;
@@ -174,7 +199,8 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
   a.walk_values = r.d_walk_values;
   a.walk_steps = r.d_walk_steps;
   char* ws = static_cast<char*>(r.d_workspace);
-  const int64_t units = a.total_units;
+  // unit arrays: one entry per unit, or per point with packed units
+  const int64_t units = static_cast<int64_t>(r.n_points) * a.units_per_point;
   a.next_unit = reinterpret_cast<int*>(ws);
   a.unit_mean = reinterpret_cast<double*>(ws + 256);
   a.unit_m2 = a.unit_mean + units;
@@ -190,13 +216,22 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
   a.first_ovf = first_ovf;
 
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, a.next_unit);
+  if (a.pack_shift >= 0) {
+    // packed units accumulate step and flag counts per point
+    const int zb = 256;
+    hipLaunchKernelGGL(k_zero_u32, dim3((r.n_points + zb - 1) / zb), dim3(zb), 0, stream, a.unit_steps,
+                       r.n_points);
+    hipLaunchKernelGGL(k_zero_u32, dim3((r.n_points + zb - 1) / zb), dim3(zb), 0, stream, a.unit_flags,
+                       r.n_points);
+  }
   launch_closest_point(bvh, r.d_points, r.n_points, first_d2, first_cp, first_tri, first_ovf, stream);
   constexpr int kSlots = 8;
   const bool poisson = r.source == N2WOS_SOURCE_POISSON_SINES;
   const int cap = tuning_value("ring", 8);
-  int grid = wos_full_grid(r.source);
+  const bool ext = r.ext != 0;
+  int grid = wos_full_grid(r.source, ext);
   if (r.max_blocks > 0) grid = std::min(grid, r.max_blocks);
-  const int64_t needed = (units + kWaves - 1) / kWaves;
+  const int64_t needed = (static_cast<int64_t>(a.total_units) + kWaves - 1) / kWaves;
   grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(grid, needed)));
   a.visits = r.d_visits;
   a.refill_min = tuning_value("refill_min", 2);
@@ -212,9 +247,20 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
   a.nc_xm = r.nc_xm;
   a.nc_need = r.nc_need;
   a.nc_full = r.nc_full;
+  a.cv_values = r.cv_values;
+  a.cv_res = r.cv_res;
+  a.cv_points = r.cv_points;
+  for (int k = 0; k < 3; ++k) {
+    a.cv_origin[k] = r.cv_origin[k];
+    a.cv_inv_h[k] = r.cv_inv_h[k];
+  }
+  a.cv_beta = r.cv_beta;
+  a.path_pos = r.path_pos;
+  a.path_q = r.path_q;
+  a.path_len = r.path_len;
   if (r.ev_begin) N2_HIP(hipEventRecord(static_cast<hipEvent_t>(r.ev_begin), stream));
-  if (poisson) launch_variant<kSlots, true>(a, grid, stream, cap == 16 ? 16 : 8);
-  else launch_variant<kSlots, false>(a, grid, stream, cap == 16 ? 16 : 8);
+  if (poisson) launch_variant<kSlots, true>(a, grid, stream, cap == 16 ? 16 : 8, ext);
+  else launch_variant<kSlots, false>(a, grid, stream, cap == 16 ? 16 : 8, ext);
   if (r.ev_end) N2_HIP(hipEventRecord(static_cast<hipEvent_t>(r.ev_end), stream));
   N2_HIP(hipGetLastError());
   const int mb = 256;
@@ -402,6 +448,80 @@ extern "C" int n2wos_wos_solve_noalloc(const n2wos_bvh* bvh, const n2wos_boundar
     // of the workspace is written before it is read
     return wos_solve_impl(bvh, boundary, wp, d_points, n_points, walks_per_point, point_offset, d_n, d_mean, d_m2,
                           d_steps, d_forced, accumulate, scratch_ptr(), d_walk_values, d_walk_steps, stream);
+  });
+}
+
+extern "C" int n2wos_cv_solve(const n2wos_bvh* bvh, const n2wos_boundary* boundary, const n2wos_walk_params* wp,
+                              const float* d_points, int n_points, int walks_per_point, uint32_t point_offset,
+                              const n2wos_cv_field* field, double* d_n, double* d_mean, double* d_m2,
+                              uint64_t* d_steps, uint32_t* d_forced, int accumulate, void* d_workspace,
+                              int64_t workspace_bytes, float* d_walk_values, int* d_walk_steps, float* d_path_pos,
+                              float* d_path_q, int path_len, void* stream) {
+  return guarded([&]() -> int {
+    check_solve_args(bvh, boundary, wp, d_points, n_points, walks_per_point, point_offset, d_n, d_mean, d_m2,
+                     d_steps, d_forced, d_walk_values, d_walk_steps);
+    N2_REQUIRE(path_len >= 0 && path_len <= 4096, N2WOS_ERR_INVALID, "path_len must be in [0, 4096]");
+    const int points = field ? field->points : 0;
+    N2_REQUIRE(points == 0 || points == 2 || points == 6, N2WOS_ERR_INVALID, "field points must be 0, 2 or 6");
+    if (points > 0) {
+      N2_REQUIRE(field->d_values, N2WOS_ERR_NULL, "NULL field values");
+      N2_REQUIRE(field->res >= 1 && field->res <= 1024, N2WOS_ERR_INVALID, "field res must be in [1, 1024]");
+      for (int k = 0; k < 3; ++k)
+        N2_REQUIRE(std::isfinite(field->origin[k]) && std::isfinite(field->cell_inverse[k]) &&
+                       field->cell_inverse[k] > 0.0f,
+                   N2WOS_ERR_INVALID, "field box must be finite with positive cell_inverse");
+      N2_REQUIRE(std::isfinite(field->beta), N2WOS_ERR_INVALID, "field beta must be finite");
+    }
+    if (path_len > 0)
+      N2_REQUIRE(d_path_pos && d_path_q && d_walk_values, N2WOS_ERR_NULL,
+                 "a path record needs d_path_pos, d_path_q and the per-walk outputs");
+    if (n_points == 0) return N2WOS_OK;
+    N2_REQUIRE(d_workspace, N2WOS_ERR_NULL, "NULL device pointer");
+    N2_REQUIRE(workspace_bytes >= wos_workspace_bytes(n_points, walks_per_point), N2WOS_ERR_INVALID,
+               "workspace too small (n2wos_wos_workspace_size)");
+    check_walk_params(wp);
+    const Boundary bc = to_boundary(boundary);
+    WosRun r{};
+    r.d_points = d_points;
+    r.n_points = n_points;
+    r.walks_per_point = walks_per_point;
+    r.point_offset = point_offset;
+    r.max_steps = wp->max_steps;
+    r.epsilon = wp->epsilon;
+    r.step_scale = wp->step_scale;
+    r.seed = wp->seed;
+    r.rng_stream = wp->rng_stream;
+    r.source = wp->source;
+    r.max_blocks = wp->max_blocks;
+    r.d_n = d_n;
+    r.d_mean = d_mean;
+    r.d_m2 = d_m2;
+    r.d_steps = d_steps;
+    r.d_forced = d_forced;
+    r.accumulate = accumulate;
+    r.d_workspace = d_workspace;
+    r.d_walk_values = d_walk_values;
+    r.d_walk_steps = d_walk_steps;
+    r.d_visits = nullptr;
+    r.mode = kModePlain;
+    r.ev_begin = wp->event_begin;
+    r.ev_end = wp->event_end;
+    r.ext = 1;
+    r.cv_points = points;
+    r.cv_beta = points > 0 ? field->beta : 0.0f;
+    if (points > 0) {
+      r.cv_values = field->d_values;
+      r.cv_res = field->res;
+      for (int k = 0; k < 3; ++k) {
+        r.cv_origin[k] = field->origin[k];
+        r.cv_inv_h[k] = field->cell_inverse[k];
+      }
+    }
+    r.path_pos = d_path_pos;
+    r.path_q = d_path_q;
+    r.path_len = path_len;
+    run_wos(bvh, bc, r, static_cast<hipStream_t>(stream));
+    return N2WOS_OK;
   });
 }
 

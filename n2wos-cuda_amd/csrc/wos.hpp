@@ -52,6 +52,26 @@ struct WosKernelArgs {
   float* nc_xm;           // per walk: X_m (3 floats)
   uint8_t* nc_need;       // per walk: 1 if the cache must be added at X_m
   float* nc_full;         // coupled mode: the full walk's value
+  // extended walker only (plain mode): along-walk control variate over the
+  // field cv_values ((cv_res + 1)^3 vertex values, cv_points 0 = off, 2 or 6)
+  const float* cv_values;
+  int cv_res;
+  int cv_points;
+  float cv_origin[3];
+  float cv_inv_h[3];
+  float cv_beta;
+  // extended walker only: per walk, the first path_len visited positions X_k
+  // and q_k = S_k - beta (control-variate terms of steps < k), so that the
+  // walk's value minus q_k estimates the solution at X_k
+  float* path_pos;
+  float* path_q;
+  int path_len;
+  // extended walker only: packed units (pack_shift >= 0, walks_per_point =
+  // 2^pack_shift <= 32): unit u holds the walks of points
+  // [u 64 / wpp, (u + 1) 64 / wpp); unit arrays are indexed by point and the
+  // step and flag counts are accumulated there (zeroed before the launch)
+  int pack_shift;
+  int n_points;
 };
 
 struct WosRun {
@@ -86,6 +106,16 @@ struct WosRun {
   float* nc_xm;
   uint8_t* nc_need;
   float* nc_full;
+  int ext;  // extended walker (control variate / path record)
+  const float* cv_values;
+  int cv_res;
+  int cv_points;
+  float cv_origin[3];
+  float cv_inv_h[3];
+  float cv_beta;
+  float* path_pos;
+  float* path_q;
+  int path_len;
 };
 
 int64_t wos_workspace_bytes(int n_points, int walks_per_point);
@@ -93,7 +123,7 @@ int64_t wos_workspace_bytes(int n_points, int walks_per_point);
 Boundary to_boundary(const n2wos_boundary* boundary);
 void check_walk_params(const n2wos_walk_params* wp);
 // Blocks of a full-device walker launch (CUs x resident blocks per CU).
-int wos_full_grid(int source);
+int wos_full_grid(int source, bool ext = false);
 // Enqueues the stand-alone closest-point kernel (bvh.hip).
 void launch_closest_point(const n2wos_bvh* bvh, const float* d_points, int n, float* d_d2, float* d_cp, int* d_tri,
                           int* d_ovf, hipStream_t stream);

@@ -79,6 +79,9 @@ def lib():
         L.orc_walks.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_int, _f32p, C.c_int, C.c_uint64,
                                 C.c_uint32, _f32p, C.c_void_p, C.c_long, C.c_int, C.c_int, _f32p, _i32p,
                                 C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+        L.orc_walks_cv.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_int, _f32p, C.c_int, C.c_uint64,
+                                   C.c_uint32, _f32p, C.c_long, C.c_int, _f32p, _i32p, C.c_int, C.c_int, C.c_int,
+                                   _f32p, _f32p, _f32p, C.c_float]
         L.orc_welford.argtypes = [_f32p, C.c_long, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_chan_merge.argtypes = [C.c_double] * 6 + [_f64p]
         L.orc_point_stats.argtypes = [_f32p, C.c_long, C.c_int, _f64p, _f64p, _f64p]
@@ -228,6 +231,23 @@ class Bvh2:
         if rc != 0:
             raise ValueError("orc_walks failed")
         return values, steps, status, xm
+
+    def walks_cv(self, pts, wpp, field, *, points=2, beta=1.0, max_steps=256, eps=1e-4, step_scale=0.999,
+                 bmode=BC_CONSTANT_ONE, bparams=None, poisson=False, seed=0, stream=0, nthreads=1):
+        """Plain walks with the along-walk control variate; field = (grid
+        values[(G+1)^3], G, origin[3], inverse cell size[3])."""
+        grid, res, origin, inv_h = field
+        pts = _f32(pts).reshape(-1, 3)
+        npts = pts.shape[0]
+        prm = _f32(bparams if bparams is not None else np.zeros(1))
+        values = np.zeros(npts * wpp, np.float32)
+        steps = np.zeros(npts * wpp, np.int32)
+        rc = lib().orc_walks_cv(self.h, max_steps, eps, step_scale, bmode, prm, int(poisson), seed, stream, pts, npts,
+                                wpp, values, steps, nthreads, points, res, _f32(grid).ravel(), _f32(origin),
+                                _f32(inv_h), beta)
+        if rc != 0:
+            raise ValueError("orc_walks_cv failed")
+        return values, steps
 
 
 def point_stats(values, npts: int, wpp: int):

@@ -737,7 +737,75 @@ typedef struct {
   uint32_t seed_lo, seed_hi;
   uint32_t stream;
   int antithetic; /* walks 2j+1 take the negated step-0 draws of walk 2j */
+  /* along-walk control variate (plain walks only; cv_points 0 = off) */
+  int cv_points;         /* 2: antipodal pair; 6: rotated octahedron */
+  int cv_res;            /* G cells per axis, (G+1)^3 vertex values */
+  const float* cv_grid;
+  float cv_origin[3], cv_inv_h[3];
+  float cv_beta;
 } orc_walk_opts;
+
+/* Control-variate field: C at the (G+1)^3 vertices of a box (origin, inverse
+ * cell size per axis), vertex (x, y, z) at index x + (G+1) (y + (G+1) z),
+ * read by trilinear interpolation in FP32 (coordinates clamped to the box). */
+static inline float lerp_f32(float a, float b, float t) { return a + t * (b - a); }
+
+static float cv_lookup(const orc_walk_opts* o, float qx, float qy, float qz) {
+  const int G = o->cv_res;
+  const float q[3] = {qx, qy, qz};
+  int i[3];
+  float f[3];
+  for (int a = 0; a < 3; ++a) {
+    const float sa = fminf(fmaxf((q[a] - o->cv_origin[a]) * o->cv_inv_h[a], 0.0f), (float)G);
+    int ia = (int)floorf(sa);
+    ia = ia < G - 1 ? ia : G - 1;
+    i[a] = ia;
+    f[a] = sa - (float)ia;
+  }
+  const long sy = G + 1, sz = (long)(G + 1) * (G + 1);
+  const float* v = o->cv_grid + i[0] + sy * i[1] + sz * i[2];
+  const float c00 = lerp_f32(v[0], v[1], f[0]);
+  const float c10 = lerp_f32(v[sy], v[sy + 1], f[0]);
+  const float c01 = lerp_f32(v[sz], v[sz + 1], f[0]);
+  const float c11 = lerp_f32(v[sz + sy], v[sz + sy + 1], f[0]);
+  const float c0 = lerp_f32(c00, c10, f[1]);
+  const float c1 = lerp_f32(c01, c11, f[1]);
+  return lerp_f32(c0, c1, f[2]);
+}
+
+/* Orthonormal frame (d, e1, e2), e1 turned by the angle 2 pi psi about d:
+ * the branch-free basis of Duff et al., "Building an Orthonormal Basis,
+ * Revisited" (JCGT 6(1), 2017), then the rotation.  With d uniform on the
+ * sphere and psi uniform, each of +-d, +-e1, +-e2 is uniform on the sphere. */
+static void cv_frame(v3 d, float psi, v3* e1, v3* e2) {
+  const float sg = copysignf(1.0f, d.z);
+  const float a = -1.0f / (sg + d.z);
+  const float b = d.x * d.y * a;
+  const v3 b1 = mk(1.0f + sg * d.x * d.x * a, sg * b, -sg * d.x);
+  const v3 b2 = mk(b, sg + d.y * d.y * a, -d.y);
+  float s, c;
+  sincos2pi(psi, &s, &c);
+  *e1 = mk(c * b1.x + s * b2.x, c * b1.y + s * b2.y, c * b1.z + s * b2.z);
+  *e2 = mk(c * b2.x - s * b1.x, c * b2.y - s * b1.y, c * b2.z - s * b1.z);
+}
+
+/* One step's control-variate term C(p + R d) - (1/m) sum_j C(p + R s_j) over
+ * the design {d, -d} (m = 2) or {d, -d, e1, -e1, e2, -e2} (m = 6): zero mean
+ * for any C, because every design point is uniform on the sphere of radius R
+ * (Walk-on-Spheres' next position is p + R d). */
+static float cv_term(const orc_walk_opts* o, v3 p, float R, v3 d, uint32_t psi_word) {
+  const float a = cv_lookup(o, p.x + R * d.x, p.y + R * d.y, p.z + R * d.z);
+  float sum = a + cv_lookup(o, p.x - R * d.x, p.y - R * d.y, p.z - R * d.z);
+  if (o->cv_points == 6) {
+    v3 e1, e2;
+    cv_frame(d, u01(psi_word), &e1, &e2);
+    sum = sum + cv_lookup(o, p.x + R * e1.x, p.y + R * e1.y, p.z + R * e1.z);
+    sum = sum + cv_lookup(o, p.x - R * e1.x, p.y - R * e1.y, p.z - R * e1.z);
+    sum = sum + cv_lookup(o, p.x + R * e2.x, p.y + R * e2.y, p.z + R * e2.z);
+    sum = sum + cv_lookup(o, p.x - R * e2.x, p.y - R * e2.y, p.z - R * e2.z);
+  }
+  return a - sum / (float)o->cv_points;
+}
 
 static inline float median3(float a, float b, float c) {
   const float mx = fmaxf(fmaxf(a, b), c), mn = fminf(fminf(a, b), c);
@@ -757,6 +825,8 @@ static inline v3 dir_from(uint32_t u0, uint32_t u1) {
 static int walk_one(const orc_bvh2* b, const orc_walk_opts* o, v3 p, uint32_t point, uint32_t walk, int depth,
                     const float* first_d2, const v3* first_cp, float* value, int* steps, v3* xm) {
   float src = 0.0f;
+  float cvs = 0.0f; /* control-variate sum (plain walks with cv_points) */
+  const int cv = o->cv_points > 0 && depth < 0;
   for (int step = 0; step <= o->max_steps; ++step) {
     float d2; v3 P; int tri;
     if (step == 0 && first_d2) {
@@ -772,6 +842,7 @@ static int walk_one(const orc_bvh2* b, const orc_walk_opts* o, v3 p, uint32_t po
     const int forced = step >= o->max_steps;
     if (d <= o->eps || forced || !isfinite(d)) {
       *value = src + boundary_f32(o->bmode, o->bparams, P);
+      if (cv) *value = *value - o->cv_beta * cvs;
       *steps = step;
       *xm = p;
       return forced ? 1 : 0;
@@ -793,10 +864,12 @@ static int walk_one(const orc_bvh2* b, const orc_walk_opts* o, v3 p, uint32_t po
     v3 dir = dir_from(r[0], r[1]);
     dir = mk(sgn * dir.x, sgn * dir.y, sgn * dir.z);
     const float R = o->step_scale * d;
+    uint32_t psi_word = r[2]; /* the frame angle: word 2 (Laplace) or word 7 (Poisson) */
     if (o->poisson) {
       const uint32_t ctr2[4] = {(uint32_t)step, cw, point, 2u * o->stream + 1u};
       uint32_t t[4];
       orc_philox4x32_10(ctr2, key, t);
+      psi_word = t[3];
       const float rho = median3(u01(r[2]), u01(r[3]), u01(t[0]));
       v3 e = dir_from(t[1], t[2]);
       e = mk(sgn * e.x, sgn * e.y, sgn * e.z);
@@ -804,9 +877,10 @@ static int walk_one(const orc_bvh2* b, const orc_walk_opts* o, v3 p, uint32_t po
       const v3 y = mk(p.x + Rr * e.x, p.y + Rr * e.y, p.z + Rr * e.z);
       src = src + (R * R) * poisson_source_f32(y) / 6.0f;
     }
+    if (cv) cvs = cvs + cv_term(o, p, R, dir, psi_word);
     p = mk(p.x + R * dir.x, p.y + R * dir.y, p.z + R * dir.z);
   }
-  *value = src;
+  *value = cv ? src - o->cv_beta * cvs : src;
   *steps = o->max_steps;
   *xm = p;
   return 1;
@@ -876,12 +950,49 @@ int orc_walks(const orc_bvh2* b, int max_steps, float eps, float step_scale, int
               int wpp, int depth, float* values, int* steps, int* status, float* xm, int nthreads, int antithetic) {
   if (!b || wpp < 1 || npts < 0) return -1;
   orc_walk_opts o = {max_steps, eps, step_scale, bmode, bparams, poisson, (uint32_t)seed, (uint32_t)(seed >> 32), stream,
-                     antithetic};
+                     antithetic, 0, 0, NULL, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, 0.0f};
   if (nthreads < 1) nthreads = 1;
   if (nthreads > 256) nthreads = 256;
   wk_shared sh;
   sh.b = b; sh.o = &o; sh.pts = pts; sh.point_ids = point_ids; sh.npts = npts; sh.wpp = wpp; sh.depth = depth;
   sh.values = values; sh.steps = steps; sh.status = status; sh.xm = xm;
+  sh.first_d2 = (float*)malloc(sizeof(float) * (size_t)(npts > 0 ? npts : 1));
+  sh.first_cp = (v3*)malloc(sizeof(v3) * (size_t)(npts > 0 ? npts : 1));
+  sh.total = npts * wpp;
+  sh.next_point = 0;
+  sh.next_sample = 0;
+  wk_shared* ptrs[256];
+  for (int t = 0; t < nthreads; ++t) ptrs[t] = &sh;
+  run_threads(wk_worker, ptrs, sizeof(wk_shared*), nthreads);
+  run_threads(wk_worker2, ptrs, sizeof(wk_shared*), nthreads);
+  free(sh.first_d2);
+  free(sh.first_cp);
+  return 0;
+}
+
+/* Plain walks with the along-walk control variate (cv_points 2 or 6, field
+ * grid[(G+1)^3] over the box origin + [0, G / inv_h)): each walk's value is
+ * W - beta sum_k [C(X_k + R_k d_k) - (1/m) sum_j C(X_k + R_k s_j)]. */
+int orc_walks_cv(const orc_bvh2* b, int max_steps, float eps, float step_scale, int bmode, const float* bparams,
+                 int poisson, uint64_t seed, uint32_t stream, const float* pts, long npts, int wpp, float* values,
+                 int* steps, int nthreads, int cv_points, int cv_res, const float* grid, const float* origin,
+                 const float* inv_h, float beta) {
+  if (!b || wpp < 1 || npts < 0 || (cv_points != 2 && cv_points != 6) || cv_res < 1 || !grid) return -1;
+  orc_walk_opts o = {max_steps, eps, step_scale, bmode, bparams, poisson, (uint32_t)seed, (uint32_t)(seed >> 32), stream,
+                     0, 0, 0, NULL, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, 0.0f};
+  o.cv_points = cv_points;
+  o.cv_res = cv_res;
+  o.cv_grid = grid;
+  for (int a = 0; a < 3; ++a) {
+    o.cv_origin[a] = origin[a];
+    o.cv_inv_h[a] = inv_h[a];
+  }
+  o.cv_beta = beta;
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  wk_shared sh;
+  sh.b = b; sh.o = &o; sh.pts = pts; sh.point_ids = NULL; sh.npts = npts; sh.wpp = wpp; sh.depth = -1;
+  sh.values = values; sh.steps = steps; sh.status = NULL; sh.xm = NULL;
   sh.first_d2 = (float*)malloc(sizeof(float) * (size_t)(npts > 0 ? npts : 1));
   sh.first_cp = (v3*)malloc(sizeof(v3) * (size_t)(npts > 0 ? npts : 1));
   sh.total = npts * wpp;

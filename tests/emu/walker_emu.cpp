@@ -123,12 +123,20 @@ namespace {
 
 n2::WosKernelArgs g_args;
 bool g_poisson = false;
+bool g_ext = false;          // run the extended walker (control variate / path record)
+n2::WosKernelArgs g_ext_args{};  // its control-variate and path fields
 
 void lane_main() {
-  if (g_poisson)
+  if (g_ext) {
+    if (g_poisson)
+      n2::k_wos_persistent<8, 8, true, false, true>(g_args);
+    else
+      n2::k_wos_persistent<8, 8, false, false, true>(g_args);
+  } else if (g_poisson) {
     n2::k_wos_persistent<8, 8, true, true>(g_args);
-  else
+  } else {
     n2::k_wos_persistent<8, 8, false, true>(g_args);
+  }
   emu::lanes[emu::cur].op = emu::kDone;
   swapcontext(&emu::lanes[emu::cur].ctx, &emu::sched);
 }
@@ -219,6 +227,29 @@ extern "C" int emu_walker(const void* nodes, const void* tris, float delta, cons
   a.weight_node = w_node;
   a.weight_tri = w_tri;
   a.weight_update = w_update;
+  a.n_points = n_points;
+  a.pack_shift = -1;
+  if (g_ext) {
+    // packed units for wpp = 1, 2, 4, ..., 32, as the host launch sets them up
+    for (int k = 0; k <= 5; ++k)
+      if (wpp == (1 << k)) a.pack_shift = k;
+    if (a.pack_shift >= 0) {
+      const int ppu = 64 >> a.pack_shift;
+      a.total_units = (n_points + ppu - 1) / ppu;
+      for (int i = 0; i < n_points; ++i) unit_steps[i] = unit_flags[i] = 0u;
+    }
+    a.cv_values = g_ext_args.cv_values;
+    a.cv_res = g_ext_args.cv_res;
+    a.cv_points = g_ext_args.cv_points;
+    for (int k = 0; k < 3; ++k) {
+      a.cv_origin[k] = g_ext_args.cv_origin[k];
+      a.cv_inv_h[k] = g_ext_args.cv_inv_h[k];
+    }
+    a.cv_beta = g_ext_args.cv_beta;
+    a.path_pos = g_ext_args.path_pos;
+    a.path_q = g_ext_args.path_q;
+    a.path_len = g_ext_args.path_len;
+  }
   g_args = a;
   g_poisson = poisson != 0;
   emu::iter_limit = iter_limit;
@@ -227,6 +258,25 @@ extern "C" int emu_walker(const void* nodes, const void* tris, float delta, cons
   emu::diverged = 0;
   for (int k = 0; k < 3; ++k) emu::path_iters[k] = emu::path_lanes[k] = 0;
   return run_wave();
+}
+
+// Selects the extended walker for the next emu_walker calls (points 0 and
+// path_len 0: the plain walker again).
+extern "C" void emu_set_ext(const float* values, int res, int points, const float* origin, const float* inv_h,
+                            float beta, float* path_pos, float* path_q, int path_len) {
+  g_ext = points > 0 || path_len > 0;
+  g_ext_args = n2::WosKernelArgs{};
+  g_ext_args.cv_values = values;
+  g_ext_args.cv_res = res;
+  g_ext_args.cv_points = points;
+  for (int k = 0; k < 3; ++k) {
+    g_ext_args.cv_origin[k] = origin ? origin[k] : 0.0f;
+    g_ext_args.cv_inv_h[k] = inv_h ? inv_h[k] : 0.0f;
+  }
+  g_ext_args.cv_beta = beta;
+  g_ext_args.path_pos = path_pos;
+  g_ext_args.path_q = path_q;
+  g_ext_args.path_len = path_len;
 }
 
 extern "C" long emu_iterations() { return emu::iters; }

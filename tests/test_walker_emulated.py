@@ -40,11 +40,12 @@ def emu():
     E.emu_walker.restype = i
     E.emu_set_tri_count.argtypes = [C.c_long]
     E.emu_tri_oob.restype = C.c_long
+    E.emu_set_ext.argtypes = [vp, i, i, vp, vp, f, vp, vp, i]
     return E
 
 
 def _case(eng, orc, emu, mesh, npts, wpp, poisson, weights=(10, 10, 6), refill=8, eps=1e-3, seed=77,
-          build_method="median", leaf=8, pts=None):
+          build_method="median", leaf=8, pts=None, cv=None):
     from n2wos_amd import problems as P
     v, t = P.make_mesh(mesh) if isinstance(mesh, str) else mesh
     nodes, recs, delta = eng.build_host(v, t, leaf_size=leaf, build_method=build_method)
@@ -73,8 +74,13 @@ def _case(eng, orc, emu, mesh, npts, wpp, poisson, weights=(10, 10, 6), refill=8
     assert rc == 0, "emulated wave hung or diverged"
     assert emu.emu_tri_oob() == 0, "triangle-record load outside the array"
     b2 = orc.Bvh2(v, t)
-    ov, os_, _, _ = b2.walks(pts, wpp, max_steps=256, eps=eps, step_scale=0.999, bmode=bc.mode,
-                             bparams=bc.oracle_params(), poisson=poisson, seed=seed, nthreads=4)
+    if cv is None:
+        ov, os_, _, _ = b2.walks(pts, wpp, max_steps=256, eps=eps, step_scale=0.999, bmode=bc.mode,
+                                 bparams=bc.oracle_params(), poisson=poisson, seed=seed, nthreads=4)
+    else:
+        ov, os_ = b2.walks_cv(pts, wpp, cv["field"], points=cv["points"], beta=cv["beta"], max_steps=256, eps=eps,
+                              step_scale=0.999, bmode=bc.mode, bparams=bc.oracle_params(), poisson=poisson,
+                              seed=seed, nthreads=4)
     assert np.array_equal(ws, os_)
     assert np.array_equal(wv.view(np.uint32), ov.view(np.uint32))
     # unit statistics: the kernel's FP64 tree equals the oracle's fixed order
@@ -82,7 +88,9 @@ def _case(eng, orc, emu, mesh, npts, wpp, poisson, weights=(10, 10, 6), refill=8
     if wpp <= 64:
         assert np.array_equal(um, m) and np.array_equal(uq, q)
     assert us.sum() == os_.sum()
-    assert vis[0] > 0 and vis[1] > 0
+    if cv is None:
+        assert vis[0] > 0 and vis[1] > 0
+    return pts, wv, ws
 
 
 def test_emulated_walker_laplace(eng, orc, emu):
@@ -116,3 +124,54 @@ def test_emulated_walker_exterior_start_points(eng, orc, emu):
     vn, _, _ = eng.normalize_to_unit_radius(v)
     pts = np.array([[1.6, 0.0, 0.0], [0.0, -1.7, 0.3], [2.5, 2.5, -2.5], [0.3, 0.2, 0.1]], np.float32)
     _case(eng, orc, emu, (vn, t), 4, 64, False, eps=1e-3, pts=pts, leaf=2)
+
+
+def _smooth_field(v, res, seed):
+    """A smooth random field on the grid over the mesh bounds padded 5%:
+    (values, res, origin, inverse cell size)."""
+    lo, hi = v.min(0).astype(np.float64), v.max(0).astype(np.float64)
+    pad = 0.05 * (hi - lo)
+    o = (lo - pad).astype(np.float32)
+    ext = hi - lo + 2 * pad
+    inv = (res / ext).astype(np.float32)
+    ax = [o[a] + np.arange(res + 1) / inv[a] for a in range(3)]
+    X, Y, Z = np.meshgrid(ax[0], ax[1], ax[2], indexing="ij")
+    r = np.random.default_rng(seed)
+    val = np.zeros(X.shape)
+    for _ in range(5):
+        k = r.normal(size=3) * 3
+        val += r.normal() * np.sin(k[0] * X + k[1] * Y + k[2] * Z + r.uniform(0, 6.3))
+    # vertex (x, y, z) at x + (res+1) (y + (res+1) z): x fastest
+    return np.ascontiguousarray(val.transpose(2, 1, 0).ravel(), np.float32), res, o, inv
+
+
+@pytest.mark.parametrize("points,poisson,npts,wpp", [(2, False, 4, 64), (6, False, 4, 64), (6, True, 4, 64),
+                                                     (6, False, 37, 4), (2, False, 70, 1), (6, True, 5, 32)])
+def test_emulated_walker_control_variate(eng, orc, emu, points, poisson, npts, wpp):
+    """The extended walker's along-walk control variate (antipodal pair or
+    turned octahedron over a trilinear field) gives the oracle's walk values
+    bit for bit, with the plain walk's step counts; its path record starts at
+    the start point with q_0 = 0 and writes exactly steps + 1 entries.  With
+    1 to 32 walks per point the units are packed (64 / wpp points per unit,
+    the last one partial), and the per-point statistics still equal the
+    oracle's reduction order."""
+    from n2wos_amd import problems as P
+    v, t = P.make_mesh("mesh1")
+    field = _smooth_field(v, 24, points + int(poisson))
+    vals, res, o, inv = field
+    plen = 40
+    ppos = np.full(npts * wpp * plen * 3, np.nan, np.float32)
+    pq = np.full(npts * wpp * plen, np.nan, np.float32)
+    emu.emu_set_ext(vals.ctypes.data, res, points, o.ctypes.data, inv.ctypes.data, 0.75, ppos.ctypes.data,
+                    pq.ctypes.data, plen)
+    try:
+        pts, wv, ws = _case(eng, orc, emu, "mesh1", npts, wpp, poisson, eps=1e-3, seed=91,
+                            cv={"field": field, "points": points, "beta": 0.75})
+    finally:
+        emu.emu_set_ext(None, 0, 0, None, None, 0.0, None, None, 0)
+    ppos = ppos.reshape(npts * wpp, plen, 3)
+    pq = pq.reshape(npts * wpp, plen)
+    for i in range(npts * wpp):
+        assert np.array_equal(ppos[i, 0], pts[i // wpp]) and pq[i, 0] == 0.0
+        n = min(int(ws[i]) + 1, plen)
+        assert np.all(np.isfinite(pq[i, :n])) and np.all(np.isnan(pq[i, n:]))
