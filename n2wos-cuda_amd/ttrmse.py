@@ -10,7 +10,9 @@ takes Adam steps on random batches of them, then is frozen for the estimate.
 
 Several cache setups can be compared in one run (depth m, grid levels,
 labels per point, Adam steps); each reports its own training time, the cache
-error at held-out walk positions X_m, and the NC and NC + 2LMC runs.
+error at held-out walk positions X_m, and the NC and NC + 2LMC runs.  The
+control-variate setups (cv.py: the cache subtracted along the whole walk,
+unbiased for any cache) report their pipeline time and estimates the same way.
 """
 from __future__ import annotations
 
@@ -51,7 +53,10 @@ class CacheSetup:
 # pairs per point reaches RMSE 1e-2 without bias, at about the cost of plain
 # WoS.
 DEFAULT_SETUPS = (
-    CacheSetup("depth4_levels16", depth=4, train_steps=200, nc_wpp=(64, 256), lmc=((256, 16), (512, 16))),
+    # (1024, 384): the optimal allocation n_c / n_r = sqrt(V_c c_r / (V_r c_c))
+    # (docs/wos_wavefront_policy.md:66-83) with V_c 0.031, V_r 0.026 and a
+    # depth-4 prefix at about 1/6 of a full walk's cost: about 2.7
+    CacheSetup("depth4_levels16", depth=4, train_steps=200, nc_wpp=(64, 256), lmc=((256, 16), (1024, 384))),
     CacheSetup("depth1_levels2_anti", depth=1, cache_kw={"n_levels": 2}, train_pairs=16, label_stride=8,
                train_steps=60, batch=1 << 15, normalize_targets=True, antithetic=True, nc_wpp=(24, 32), lmc=()),
     CacheSetup("depth1_levels2_anti_more_labels", depth=1, cache_kw={"n_levels": 2}, train_pairs=16,
@@ -62,8 +67,108 @@ DEFAULT_SETUPS = (
 )
 
 
+@dataclass
+class CvSetup:
+    """The learned cache as an along-walk control variate (unbiased): labels
+    from the path records of one walk per labelled point, Adam steps, a bake
+    of the cache onto a grid, then `rounds` bootstrap rounds (labels from
+    control-variate walks with the current field, which are far less noisy,
+    and more Adam steps), then the estimate with wpp walks per point."""
+    name: str
+    cache_kw: dict = field(default_factory=dict)
+    label_stride: int = 2
+    path_len: int = 32
+    rounds: int = 1
+    train_steps: tuple = (40, 40)  # Adam steps per round (round 0 = plain labels)
+    batch: int = 1 << 15
+    res: int = 64                  # grid cells per axis of the baked field
+    points: int = 6                # design: 2 (antipodal) or 6 (turned octahedron)
+    wpp: tuple = (1, 2, 4)
+
+
+# A 3-level dense grid (N_min 8, growth 2) with the 4 x 64 MLP; labels on every
+# second grid point; one bootstrap round.  Measured on the benchmark problem:
+# RMSE 0.0094 with 2 walks per point in about 0.09 s, training included.
+DEFAULT_CV_SETUPS = (
+    CvSetup("cv_levels3_boot1", cache_kw={"n_levels": 3, "base_resolution": 8, "growth": 2.0}),
+    CvSetup("cv_levels3_boot2", cache_kw={"n_levels": 3, "base_resolution": 8, "growth": 2.0}, rounds=2,
+            train_steps=(40, 40, 40), wpp=(1, 2)),
+    # the antipodal pair instead of the octahedron, for comparison
+    CvSetup("cv_levels3_boot1_pair", cache_kw={"n_levels": 3, "base_resolution": 8, "growth": 2.0}, points=2,
+            wpp=(2, 8)),
+)
+
+
+def run_cv(bvh, verts, points, boundary, truth, params, plain_time, tau=1e-2, setups=DEFAULT_CV_SETUPS, seed=11,
+           log=print):
+    """Time-to-RMSE of the control-variate estimator: each setup's training
+    pipeline is timed once; each estimate adds its own solve time."""
+    import torch
+
+    from . import cache as K, cv as CV
+    tp = torch.from_numpy(points).cuda()
+    lo, hi = verts.min(0), verts.max(0)
+    cs = CV.CvSolver(bvh, boundary, params)
+    cs.solve(tp[:1024].contiguous(), 1, None, path_len=4)  # warm the extended walker
+    out = []
+    best = None
+    for su in setups:
+        res = {"name": su.name, "cache": dict(su.cache_kw), "label_stride": su.label_stride,
+               "path_len": su.path_len, "rounds": su.rounds, "train_steps": list(su.train_steps),
+               "batch": su.batch, "field_res": su.res, "design_points": su.points}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cache = K.Cache(K.CacheConfig.for_bounds(lo, hi, seed=seed, **su.cache_kw))
+        lp = tp[:: su.label_stride].contiguous()
+        st = cs.solve(lp, 1, None, path_len=su.path_len, seed=seed + 101, stream_id=5)
+        x, y = CV.CvSolver.path_labels(st)
+        cache.set_output_offset(float(y.mean().item()))
+        gen = torch.Generator(device="cuda").manual_seed(seed)
+        nlab = []
+        fld = None
+        for r in range(su.rounds + 1):
+            nlab.append(int(x.shape[0]))
+            for _ in range(su.train_steps[r]):
+                idx = torch.randint(0, x.shape[0], (su.batch,), device="cuda", generator=gen)
+                cache.train_step(x[idx].contiguous(), y[idx].contiguous())
+            fld = CV.bake(cache, lo, hi, res=su.res, points=su.points)
+            if r < su.rounds:
+                st = cs.solve(lp, 1, fld, path_len=su.path_len, seed=seed + 200 + r, stream_id=6 + r)
+                x, y = CV.CvSolver.path_labels(st)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        res["train_seconds"] = t_train
+        res["labels_per_round"] = nlab
+        log(f"[{su.name}] pipeline (labels, {su.rounds} bootstrap rounds, training, bake) {t_train:.4f}s")
+        runs = []
+        for wpp in su.wpp:
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            st = cs.solve(tp, wpp, fld, seed=seed + 1000 + wpp)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t1
+            mean = st.mean.cpu().numpy()
+            var = float(st.variance().mean().item()) if wpp > 1 else None
+            r_ = _rmse(mean, truth)
+            runs.append({"wpp": wpp, "seconds": dt, "seconds_with_training": dt + t_train, "rmse": r_,
+                         "mean_walk_variance": var, "mean_error": float(np.mean(mean - truth))})
+            log(f"[{su.name}] cv wpp={wpp} t={dt:.4f}s (+train {t_train:.4f}s) rmse={r_:.4g} var={var}")
+        res["runs"] = runs
+        ok = [p["seconds_with_training"] for p in runs if p["rmse"] <= tau]
+        v = min(ok) if ok else None
+        res["time_to_rmse"] = v
+        res["speedup_vs_plain"] = (plain_time / v) if (plain_time and v) else None
+        if v is not None and (best is None or v < best[0]):
+            best = (v, su.name)
+        out.append(res)
+        del cache
+    return {"setups": out, "best_time_to_rmse": best[0] if best else None,
+            "best_setup": best[1] if best else None,
+            "best_speedup_vs_plain": (plain_time / best[0]) if (plain_time and best) else None}
+
+
 def run(bvh, verts, points, boundary, truth, params, tau=1e-2, plain_wpp=(256, 320, 384, 448, 512),
-        setups=DEFAULT_SETUPS, seed=11, log=print):
+        setups=DEFAULT_SETUPS, seed=11, log=print, cv_setups=DEFAULT_CV_SETUPS):
     import torch
 
     from . import cache as K, nc as N, wos as W
@@ -194,4 +299,11 @@ def run(bvh, verts, points, boundary, truth, params, tau=1e-2, plain_wpp=(256, 3
         out[f"best_{k}_time_to_rmse"] = v[0] if v else None
         out[f"best_{k}_setup"] = v[1] if v else None
         out[f"best_{k}_speedup_vs_plain"] = (pt / v[0]) if (pt and v) else None
+    if cv_setups:
+        # the learned cache as an along-walk control variate: unbiased
+        cvr = run_cv(bvh, verts, points, boundary, truth, params, pt, tau, cv_setups, seed, log)
+        out["cv"] = cvr
+        out["best_unbiased_time_to_rmse"] = cvr["best_time_to_rmse"]
+        out["best_unbiased_setup"] = cvr["best_setup"]
+        out["best_unbiased_speedup_vs_plain"] = cvr["best_speedup_vs_plain"]
     return out
