@@ -342,6 +342,8 @@ typedef struct {
   float beta1, beta2, epsilon;
   float loss_scale;      /* FP16 gradient scale */
   uint64_t seed;         /* parameter initialisation */
+  int max_batch;         /* largest training batch (0: 262144); its activation
+                            store (1,090 B per sample) is allocated here */
 } n2wos_cache_config;
 int n2wos_cache_create(const n2wos_cache_config* config, n2wos_cache** out);
 int n2wos_cache_destroy(n2wos_cache* cache);
@@ -360,8 +362,8 @@ int n2wos_cache_copy_params(n2wos_cache* dst, const n2wos_cache* src, void* stre
 /* d_out[i] = C(d_points[i]) for n points (12 B each). */
 int n2wos_cache_infer(const n2wos_cache* cache, const float* d_points, int n, float* d_out, void* stream);
 /* One Adam step on the batch (n points, n targets).  *d_loss_out (device,
- * optional) receives the summed squared error.  The activation store grows
- * (one allocation) the first time a larger batch is seen. */
+ * optional) receives the summed squared error.  No allocation: n above the
+ * config's max_batch returns N2WOS_ERR_INVALID. */
 int n2wos_cache_train_step(n2wos_cache* cache, const float* d_points, const float* d_targets, int n,
                            float* d_loss_out, void* stream);
 /* Grid layout: per-level offsets, sizes (entries), resolutions and dense flags. */

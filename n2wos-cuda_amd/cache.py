@@ -15,7 +15,7 @@ class CacheConfigC(C.Structure):
     _fields_ = [("n_levels", C.c_int), ("log2_table_size", C.c_int), ("base_resolution", C.c_int),
                 ("growth", C.c_float), ("origin", C.c_float * 3), ("extent", C.c_float * 3),
                 ("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
-                ("loss_scale", C.c_float), ("seed", C.c_uint64)]
+                ("loss_scale", C.c_float), ("seed", C.c_uint64), ("max_batch", C.c_int)]
 
 
 @dataclass
@@ -34,6 +34,7 @@ class CacheConfig:
     epsilon: float = 1e-15
     loss_scale: float = 128.0
     seed: int = 1
+    max_batch: int = 1 << 18  # largest training batch (activation store allocated at creation)
 
     @classmethod
     def for_bounds(cls, lo, hi, pad: float = 0.05, **kw):
@@ -51,7 +52,7 @@ class CacheConfig:
             c.origin[a] = self.origin[a]
             c.extent[a] = self.extent[a]
         c.learning_rate, c.beta1, c.beta2, c.epsilon = self.learning_rate, self.beta1, self.beta2, self.epsilon
-        c.loss_scale, c.seed = self.loss_scale, self.seed
+        c.loss_scale, c.seed, c.max_batch = self.loss_scale, self.seed, self.max_batch
         return c
 
 

@@ -14,6 +14,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "api_internal.hpp"
@@ -542,6 +543,8 @@ extern "C" int n2wos_cache_create(const n2wos_cache_config* cfg, n2wos_cache** o
       c->beta2 = cfg->beta2;
       c->eps = cfg->epsilon;
       c->loss_scale = cfg->loss_scale;
+      const int max_batch = cfg->max_batch > 0 ? cfg->max_batch : (1 << 18);
+      N2_REQUIRE(max_batch <= (1 << 24), N2WOS_ERR_INVALID, "max_batch must be <= 2^24");
       const size_t gf = grid_floats(c);
       // one gradient copy per XCD for grids up to 2^20 floats (4 MB): measured
       // on a 6-level 2^13 grid, a training step takes 0.24 instead of 0.33 ms;
@@ -558,6 +561,9 @@ extern "C" int n2wos_cache_create(const n2wos_cache_config* cfg, n2wos_cache** o
       N2_HIP(hipMalloc(&c->wh, kCacheWeights * 2));
       N2_HIP(hipMalloc(&c->bad, 64));
       N2_HIP(hipMalloc(&c->loss, 64));
+      // the activation store of the largest batch: training allocates nothing
+      c->store_batch = (max_batch + 15) / 16 * 16;
+      N2_HIP(hipMalloc(&c->store, static_cast<size_t>(c->store_batch) * (kCacheIn + 8 * kCacheHidden + 1) * 2));
       N2_HIP(hipMemset(c->grid_grad, 0, gf * 4 * c->grad_copies));
       N2_HIP(hipMemset(c->grid_m, 0, gf * 4));
       N2_HIP(hipMemset(c->grid_v, 0, gf * 4));
@@ -621,14 +627,9 @@ extern "C" int n2wos_cache_train_step(n2wos_cache* c, const float* d_points, con
     N2_REQUIRE(d_points && d_targets, N2WOS_ERR_NULL, "NULL device pointer");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int batch = (n + 15) / 16 * 16;
-    if (batch > c->store_batch) {
-      // grow the activation store (the one allocation of the training path)
-      if (c->store) N2_HIP(hipFree(c->store));
-      c->store = nullptr;
-      const size_t halves = static_cast<size_t>(batch) * (kCacheIn + 8 * kCacheHidden + 1);
-      N2_HIP(hipMalloc(&c->store, halves * 2));
-      c->store_batch = batch;
-    }
+    N2_REQUIRE(batch <= c->store_batch, N2WOS_ERR_INVALID,
+               "batch of " + std::to_string(n) + " exceeds the cache's max_batch (" +
+                   std::to_string(c->store_batch) + ")");
     TrainStore st;
     st.batch = batch;
     st.x = c->store;
@@ -752,12 +753,7 @@ extern "C" int n2wos_cache_gradients(n2wos_cache* c, const float* d_points, cons
     N2_REQUIRE(c != nullptr, N2WOS_ERR_NULL, "NULL cache");
     N2_REQUIRE(n >= 1 && d_points && d_targets, N2WOS_ERR_INVALID, "bad batch");
     const int batch = (n + 15) / 16 * 16;
-    if (batch > c->store_batch) {
-      if (c->store) N2_HIP(hipFree(c->store));
-      c->store = nullptr;
-      N2_HIP(hipMalloc(&c->store, static_cast<size_t>(batch) * (kCacheIn + 8 * kCacheHidden + 1) * 2));
-      c->store_batch = batch;
-    }
+    N2_REQUIRE(batch <= c->store_batch, N2WOS_ERR_INVALID, "batch exceeds the cache's max_batch");
     TrainStore st;
     st.batch = batch;
     st.x = c->store;

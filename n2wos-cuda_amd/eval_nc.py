@@ -346,7 +346,8 @@ def main(argv=None) -> int:
     # cache and training
     cfg = K.CacheConfig.for_bounds(lo, hi, n_levels=opt.n_levels, log2_table_size=opt.log2_hashmap_size,
                                    base_resolution=opt.base_resolution, growth=opt.per_level_scale,
-                                   learning_rate=opt.learning_rate, seed=opt.seed)
+                                   learning_rate=opt.learning_rate, seed=opt.seed,
+                                   max_batch=opt.batch_size if opt.batch_size > 0 else len(train))
     cache = K.Cache(cfg)
     solver = W.Solver(bvh, bc, params)
     label_ms = train_ms = 0.0

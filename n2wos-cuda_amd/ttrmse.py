@@ -118,7 +118,7 @@ def run_cv(bvh, verts, points, boundary, truth, params, plain_time, tau=1e-2, se
                "batch": su.batch, "field_res": su.res, "design_points": su.points}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        cache = K.Cache(K.CacheConfig.for_bounds(lo, hi, seed=seed, **su.cache_kw))
+        cache = K.Cache(K.CacheConfig.for_bounds(lo, hi, seed=seed, max_batch=su.batch, **su.cache_kw))
         lp = tp[:: su.label_stride].contiguous()
         st = cs.solve(lp, 1, None, path_len=su.path_len, seed=seed + 101, stream_id=5)
         x, y = CV.CvSolver.path_labels(st)

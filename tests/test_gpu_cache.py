@@ -149,3 +149,22 @@ def test_output_offset_in_inference_and_loss(eng, torch_gpu):
     assert rel(da, db) < 1e-2
     with pytest.raises(Exception):
         a.set_output_offset(float("nan"))
+
+
+def test_training_batch_limit_is_set_at_creation(eng, torch_gpu):
+    """The activation store is sized by max_batch when the cache is created:
+    batches up to it train, a larger one is rejected before anything is
+    enqueued (no allocation inside a training step)."""
+    from n2wos_amd import cache as K
+    from n2wos_amd._lib import ERR_INVALID, N2wosError
+    torch = torch_gpu
+    c = K.Cache(K.CacheConfig(seed=5, n_levels=2, max_batch=1000))
+    g = torch.Generator(device="cuda").manual_seed(1)
+    x = (torch.rand((1008, 3), device="cuda", generator=g) * 2 - 1).contiguous()
+    y = torch.rand(1008, device="cuda", generator=g)
+    c.train_step(x[:1000].contiguous(), y[:1000].contiguous())
+    c.train_step(x, y)  # 1008 = 1000 rounded up to whole 16-sample tiles
+    with pytest.raises(N2wosError) as e:
+        c.train_step(torch.zeros((1009, 3), device="cuda"), torch.zeros(1009, device="cuda"))
+    assert e.value.code == ERR_INVALID
+    torch.cuda.synchronize()
