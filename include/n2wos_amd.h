@@ -388,7 +388,9 @@ int n2wos_cache_gradients(n2wos_cache* cache, const float* d_points, const float
  * point `first_round_walks` walks; each later round gives every point the
  * walks that its estimated RMSE, sqrt(mean_i var_i / n_i), says are still
  * missing (x 1.1, in units of 64, at most `max_round_walks`), until the
- * estimate is <= the target or `max_walks_per_point` is reached.
+ * estimate is <= the target or `max_walks_per_point` is reached.  The
+ * estimate and the next round's size are computed on the device and read
+ * back asynchronously, so a round ends without a host synchronisation.
  *
  * Continuous batching: each mesh has two batch slots, each with its own
  * stream.  A slot's launch holds pieces of the current rounds of all the
@@ -427,6 +429,7 @@ typedef struct {
   double estimated_rmse;      /* after the last completed round (-1 before) */
   uint64_t walk_steps;        /* total steps of the finished request (0 before) */
   double t_submit, t_admit, t_done;  /* seconds since server creation (-1 if not yet) */
+  int control_variate;        /* 1: the mesh's control-variate field was used */
 } n2wos_request_info;
 typedef struct {
   int64_t ticks;
@@ -435,6 +438,9 @@ typedef struct {
   uint64_t walk_steps;        /* steps of finished requests */
   int64_t arena_bytes, arena_used_bytes, fixed_bytes;
   double busy_seconds;        /* wall time inside n2wos_server_step */
+  int64_t launches;           /* batch launches so far */
+  double gpu_busy_seconds;    /* device time covered by at least one retired launch */
+  double gpu_span_seconds;    /* device time from the first launch's start to the last's end */
 } n2wos_server_stats;
 int n2wos_server_create(const n2wos_server_config* config, n2wos_server** out);
 int n2wos_server_destroy(n2wos_server* server);
@@ -443,6 +449,16 @@ int n2wos_server_destroy(n2wos_server* server);
  * buffers (counted in fixed_bytes) and its stream. */
 int n2wos_server_add_mesh(n2wos_server* server, const n2wos_bvh* bvh, const n2wos_boundary* boundary,
                           const n2wos_walk_params* params, int* mesh_id);
+/* Attaches a control-variate field to a mesh (before any request to it):
+ * its batches then run the along-walk control variate (n2wos_cv_solve) over
+ * `field`, whose device values must outlive the server, in entries of
+ * unit_walks walks (a power of two <= 64) with first rounds of
+ * first_round_walks (a multiple of unit_walks).  field NULL or points 0
+ * restores plain walks.  The caller's policy decides per mesh: attach the
+ * field when it gains over plain walks (variance reduction / cost per walk
+ * > 1), keep plain walks otherwise.  No counterpart in the reference. */
+int n2wos_server_set_mesh_field(n2wos_server* server, int mesh_id, const n2wos_cv_field* field, int unit_walks,
+                                int first_round_walks);
 /* Queues a request (host points are copied; synchronous).  Fails with
  * N2WOS_ERR_NOMEM if the request can never fit the budget. */
 int n2wos_server_submit(n2wos_server* server, int mesh_id, const float* host_points, int n_points,
@@ -451,7 +467,8 @@ int n2wos_server_submit(n2wos_server* server, int mesh_id, const float* host_poi
 int n2wos_server_step(n2wos_server* server, int* n_done);
 int n2wos_server_request_info(const n2wos_server* server, int64_t request_id, n2wos_request_info* info);
 /* Copies a finished request's per-point results to host arrays (any may be
- * NULL) and forgets the request. */
+ * NULL), frees its arena range and forgets the request.  Until then a
+ * finished request keeps its range (results wait on the device). */
 int n2wos_server_take_result(n2wos_server* server, int64_t request_id, double* n, double* mean, double* m2,
                              uint64_t* steps, uint32_t* forced);
 int n2wos_server_get_stats(const n2wos_server* server, n2wos_server_stats* stats);

@@ -21,13 +21,23 @@
 //
 // A step: admit queued requests in arrival order while the arena has room,
 // fill free slots, wait for the oldest launch, retire every completed one;
-// for requests whose round is fully merged a fixed-order block reduction
-// gives the estimated RMSE; finished requests are copied out and their arena
-// ranges freed; freed slots are refilled at once.
+// for requests whose round is fully merged a fixed-order block reduction on
+// the device gives the estimated RMSE and sizes the next round (0 = done),
+// and the host reads the decisions back asynchronously (an event it polls),
+// so no round ends in a host synchronisation; a finished request's results
+// are copied to its pinned host buffer the same way (its arena range is
+// freed when that copy's event has completed); freed slots are refilled at
+// once.
+//
+// A mesh may carry a control-variate field (n2wos_server_set_mesh_field):
+// its batches then run the extended walker with the along-walk control
+// variate, and its rounds count in units of fewer walks (packed walker
+// units), since each walk carries far less variance.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <cmath>
 #include <deque>
 #include <map>
@@ -60,7 +70,7 @@ struct ServePiece {
   int point_start;      // prefix sum of counts over the pieces before this one
   uint32_t id_base;     // the request's counter base
   uint32_t walk_base;   // first walk index of the round
-  int pad;
+  int unit;             // walks per batch entry
 };
 
 __device__ __forceinline__ int find_piece(const ServePiece* pc, int n_pieces, int i, bool by_entry) {
@@ -85,7 +95,7 @@ __global__ void k_serve_gather(const ServePiece* __restrict__ pc, int n_pieces, 
   bpts[3 * i + 1] = p.points[3 * j + 1];
   bpts[3 * i + 2] = p.points[3 * j + 2];
   brng[2 * i] = p.id_base + static_cast<uint32_t>(j);
-  brng[2 * i + 1] = p.walk_base + static_cast<uint32_t>(kUnit * c);
+  brng[2 * i + 1] = p.walk_base + static_cast<uint32_t>(p.unit * c);
 }
 
 // Merges each piece point's entries, in walk order, into the request state:
@@ -118,29 +128,64 @@ __global__ void k_serve_merge(const ServePiece* __restrict__ pc, int n_pieces, i
 struct EstItem {
   const double* n;
   const double* m2;
+  const uint64_t* steps;
+  double target;
   int count;
+  int wpp;        // walks per point merged, this round included
+  int unit;       // round granularity (walks per batch entry)
+  int max_round;  // largest next round: max_round_walks capped by the walks left
+};
+
+struct EstOut {
+  double est;      // estimated RMSE: sqrt(mean over points of var_i / n_i)
+  uint64_t steps;  // walk steps of the request so far
+  int next;        // walks per point of the next round, 0 = finished
   int pad;
 };
 
 constexpr int kEstBlock = 256;
 
-// out[b] = mean over the request's points of var_i / n_i (fixed reduction order).
+// Per request (one block): the estimated RMSE and the request's step total
+// by fixed-order block reductions, and the next round's size: the walks the
+// estimate says are still missing (variance ~ 1 / n), 10 % margin, rounded up
+// to whole units and clamped to [unit, max_round]; 0 when the estimate meets
+// the target or no walks are left.
 __global__ void __launch_bounds__(kEstBlock) k_serve_estimate(const EstItem* __restrict__ items,
-                                                              double* __restrict__ out) {
+                                                              EstOut* __restrict__ out) {
   __shared__ double s[kEstBlock];
+  __shared__ uint64_t st[kEstBlock];
   const EstItem it = items[blockIdx.x];
   double acc = 0.0;
+  uint64_t steps = 0;
   for (int j = threadIdx.x; j < it.count; j += kEstBlock) {
     const double n = it.n[j];
     if (n > 1.0) acc += it.m2[j] / (n - 1.0) / n;
+    steps += it.steps[j];
   }
   s[threadIdx.x] = acc;
+  st[threadIdx.x] = steps;
   __syncthreads();
   for (int w = kEstBlock / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    if (threadIdx.x < w) {
+      s[threadIdx.x] += s[threadIdx.x + w];
+      st[threadIdx.x] += st[threadIdx.x + w];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[blockIdx.x] = s[0] / static_cast<double>(it.count);
+  if (threadIdx.x == 0) {
+    const double est = sqrt(fmax(0.0, s[0] / static_cast<double>(it.count)));
+    int next = 0;
+    if (!(est <= it.target) && it.max_round > 0) {
+      const double ratio = (est / it.target) * (est / it.target);
+      const double want = ceil(static_cast<double>(it.wpp) * (ratio - 1.0) * 1.1);
+      int64_t w = want < 4.0e9 ? static_cast<int64_t>(want) : int64_t{4000000000};
+      w = (w + it.unit - 1) / it.unit * it.unit;
+      w = w < it.unit ? it.unit : w;
+      w = w > it.max_round ? it.max_round : w;
+      next = static_cast<int>(w);
+    }
+    out[blockIdx.x] = EstOut{est, st[0], next, 0};
+  }
 }
 
 __global__ void k_fill_zero(uint32_t* p, int64_t words) {
@@ -204,6 +249,7 @@ int64_t request_bytes(int n) {
 // One in-flight batch of a mesh: its own stream, buffers and completion event.
 struct Slot {
   hipStream_t stream = nullptr;
+  hipEvent_t begin = nullptr;  // timing: the launch's first and last commands
   hipEvent_t done = nullptr;
   bool busy = false;
   int64_t seq = 0;        // launch order (the oldest busy slot is waited on first)
@@ -229,9 +275,38 @@ struct Mesh {
   Boundary bc{};
   n2wos_walk_params wp{};
   int cap = 0;            // batch entries per slot
+  int unit = kUnit;       // walks per batch entry (a round is a multiple)
+  int first_round = 0;    // walks per point of a request's first round
+  bool cv = false;        // along-walk control variate over the field below
+  n2wos_cv_field field{};
   Slot slot[kSlotsPerMesh];
   int64_t bytes = 0;
 };
+
+// Decisions of one batch of ended rounds, read back asynchronously.
+struct EstBatch {
+  EstItem* d_items = nullptr;
+  EstItem* h_items = nullptr;  // pinned
+  EstOut* d_out = nullptr;
+  EstOut* h_out = nullptr;     // pinned
+  hipEvent_t done = nullptr;
+  std::vector<int64_t> ids;
+  bool busy = false;
+  int64_t seq = 0;
+};
+
+constexpr int kEstBatches = 4;
+
+// Result copies of finished requests (device arena -> pinned host), one event.
+struct ResBatch {
+  hipEvent_t done = nullptr;
+  std::vector<int64_t> ids;
+  bool busy = false;
+  int64_t seq = 0;
+};
+
+// Pinned host bytes of one request's results: n, mean, m2, steps (8 B each), forced.
+int64_t result_bytes(int n) { return 36 * static_cast<int64_t>(n); }
 
 struct Request {
   int64_t id = 0;
@@ -249,6 +324,7 @@ struct Request {
   uint64_t* d_steps = nullptr;
   uint32_t* d_forced = nullptr;
   int rounds = 0;
+  int unit = kUnit;       // the mesh's walks per batch entry
   int wpp = 0;            // walks merged per point
   int round_walks = 0;    // walks per point of the current round
   int cursor = 0;         // points of the current round already batched
@@ -256,9 +332,7 @@ struct Request {
   double est = -1.0;
   uint64_t total_steps = 0;
   double t_submit = -1.0, t_admit = -1.0, t_done = -1.0;
-  std::vector<double> r_n, r_mean, r_m2;
-  std::vector<uint64_t> r_steps;
-  std::vector<uint32_t> r_forced;
+  char* h_res = nullptr;  // pinned results (result_bytes), filled when the request finishes
 };
 
 }  // namespace
@@ -279,10 +353,11 @@ struct n2wos_server {
   int64_t fixed_bytes = 0;
   double busy = 0.0;
   hipStream_t control = nullptr;  // admission uploads, estimates, result downloads
-  n2::EstItem* d_est_items = nullptr;
-  n2::EstItem* h_est_items = nullptr;
-  double* d_est = nullptr;
-  double* h_est = nullptr;
+  n2::EstBatch est[n2::kEstBatches];
+  std::vector<n2::ResBatch> res;
+  int64_t est_seq = 0;
+  hipEvent_t t_ref = nullptr;     // device-time origin of the launch intervals
+  std::vector<std::pair<float, float>> intervals;  // device ms of each retired launch
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -321,7 +396,7 @@ void server_admit(n2wos_server* s) {
     N2_HIP(hipStreamSynchronize(s->control));  // the host copy of the points can go
     std::vector<float>().swap(r.host_points);
     r.state = N2WOS_REQ_ACTIVE;
-    r.round_walks = s->cfg.first_round_walks;
+    r.round_walks = s->meshes[static_cast<size_t>(r.mesh)]->first_round;
     r.cursor = 0;
     r.t_admit = s->now();
     s->active.push_back(r.id);
@@ -365,7 +440,7 @@ bool launch_slot(n2wos_server* s, int mi, Slot& sl) {
   for (size_t q = 0; q < ids.size(); ++q) {
     const Request& r = *s->requests.at(ids[q]);
     left[q] = r.n - r.cursor;
-    units[q] = r.round_walks / kUnit;
+    units[q] = r.round_walks / r.unit;
   }
   const std::vector<int> grant = fair_share(left, units, m.cap);
   int entries = 0, points = 0, np = 0;
@@ -387,7 +462,7 @@ bool launch_slot(n2wos_server* s, int mi, Slot& sl) {
     pc.point_start = points;
     pc.id_base = r.rng_base;
     pc.walk_base = static_cast<uint32_t>(r.wpp);
-    pc.pad = 0;
+    pc.unit = r.unit;
     entries += grant[q] * units[q];
     points += grant[q];
     r.cursor += grant[q];
@@ -396,6 +471,7 @@ bool launch_slot(n2wos_server* s, int mi, Slot& sl) {
   }
   if (entries == 0) return false;
   hipStream_t st = sl.stream;
+  N2_HIP(hipEventRecord(sl.begin, st));
   N2_HIP(hipMemcpyAsync(sl.d_pieces, sl.h_pieces, sizeof(ServePiece) * np, hipMemcpyHostToDevice, st));
   const int b = 256;
   hipLaunchKernelGGL(k_serve_gather, dim3((entries + b - 1) / b), dim3(b), 0, st, sl.d_pieces, np, entries, sl.bpts,
@@ -404,7 +480,7 @@ bool launch_slot(n2wos_server* s, int mi, Slot& sl) {
   WosRun w{};
   w.d_points = sl.bpts;
   w.n_points = entries;
-  w.walks_per_point = kUnit;
+  w.walks_per_point = m.unit;
   w.point_offset = 0;
   w.d_point_rng = sl.brng;
   w.max_steps = m.wp.max_steps;
@@ -422,6 +498,19 @@ bool launch_slot(n2wos_server* s, int mi, Slot& sl) {
   w.accumulate = 0;
   w.d_workspace = sl.ws;
   w.mode = kModePlain;
+  if (m.cv) {
+    // the extended walker: control variate over the mesh's field (packed
+    // units when the entry holds fewer than 64 walks)
+    w.ext = 1;
+    w.cv_values = m.field.d_values;
+    w.cv_res = m.field.res;
+    w.cv_points = m.field.points;
+    for (int k = 0; k < 3; ++k) {
+      w.cv_origin[k] = m.field.origin[k];
+      w.cv_inv_h[k] = m.field.cell_inverse[k];
+    }
+    w.cv_beta = m.field.beta;
+  }
   run_wos(m.bvh, m.bc, w, st);
   hipLaunchKernelGGL(k_serve_merge, dim3((points + b - 1) / b), dim3(b), 0, st, sl.d_pieces, np, points, sl.bn,
                      sl.bmean, sl.bm2, sl.bsteps, sl.bforced);
@@ -430,7 +519,7 @@ bool launch_slot(n2wos_server* s, int mi, Slot& sl) {
   sl.busy = true;
   sl.seq = ++s->launches;
   sl.entries = entries;
-  s->walks += static_cast<int64_t>(entries) * kUnit;
+  s->walks += static_cast<int64_t>(entries) * m.unit;
   return true;
 }
 
@@ -442,6 +531,12 @@ void launch_free_slots(n2wos_server* s) {
 
 // Retires a completed slot: requests whose round is fully merged go to `round_done`.
 void retire(n2wos_server* s, Slot& sl, std::vector<int64_t>& round_done) {
+  float a = 0.0f, b = 0.0f;
+  if (hipEventElapsedTime(&a, s->t_ref, sl.begin) == hipSuccess &&
+      hipEventElapsedTime(&b, s->t_ref, sl.done) == hipSuccess)
+    s->intervals.emplace_back(a, b);
+  else
+    (void)hipGetLastError();
   for (int64_t id : sl.reqs) {
     Request& r = *s->requests.at(id);
     r.inflight -= 1;
@@ -451,67 +546,150 @@ void retire(n2wos_server* s, Slot& sl, std::vector<int64_t>& round_done) {
   sl.busy = false;
 }
 
-int finish_rounds(n2wos_server* s, const std::vector<int64_t>& round_done) {
-  if (round_done.empty()) return 0;
-  hipStream_t st = s->control;
-  const int k = static_cast<int>(round_done.size());
-  for (int q = 0; q < k; ++q) {
-    const Request& r = *s->requests.at(round_done[q]);
-    s->h_est_items[q] = EstItem{r.d_n, r.d_m2, r.n, 0};
+// Applies the decisions of a completed estimate batch: finished requests
+// (their results stay in the arena) or the size of their next round.
+void apply_estimates(n2wos_server* s, EstBatch& eb) {
+  ResBatch* rb = nullptr;
+  for (size_t q = 0; q < eb.ids.size(); ++q) {
+    Request& r = *s->requests.at(eb.ids[q]);
+    const EstOut& o = eb.h_out[q];
+    r.est = o.est;
+    if (o.next == 0) {
+      // finished: copy the results to the request's pinned buffer
+      r.total_steps = o.steps;
+      if (!rb) {
+        for (ResBatch& x : s->res)
+          if (!x.busy) {
+            rb = &x;
+            break;
+          }
+        if (!rb) {
+          s->res.emplace_back();
+          rb = &s->res.back();
+          N2_HIP(hipEventCreateWithFlags(&rb->done, hipEventDisableTiming));
+        }
+      }
+      const int64_t p = r.n;
+      char* h = r.h_res;
+      hipStream_t st = s->control;
+      N2_HIP(hipMemcpyAsync(h, r.d_n, 8 * p, hipMemcpyDeviceToHost, st));
+      N2_HIP(hipMemcpyAsync(h + 8 * p, r.d_mean, 8 * p, hipMemcpyDeviceToHost, st));
+      N2_HIP(hipMemcpyAsync(h + 16 * p, r.d_m2, 8 * p, hipMemcpyDeviceToHost, st));
+      N2_HIP(hipMemcpyAsync(h + 24 * p, r.d_steps, 8 * p, hipMemcpyDeviceToHost, st));
+      N2_HIP(hipMemcpyAsync(h + 32 * p, r.d_forced, 4 * p, hipMemcpyDeviceToHost, st));
+      rb->ids.push_back(r.id);
+    } else {
+      r.round_walks = o.next;
+      r.cursor = 0;
+    }
   }
-  N2_HIP(hipMemcpyAsync(s->d_est_items, s->h_est_items, sizeof(EstItem) * k, hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_serve_estimate, dim3(k), dim3(kEstBlock), 0, st, s->d_est_items, s->d_est);
-  N2_HIP(hipGetLastError());
-  N2_HIP(hipMemcpyAsync(s->h_est, s->d_est, sizeof(double) * k, hipMemcpyDeviceToHost, st));
-  N2_HIP(hipStreamSynchronize(st));
+  if (rb) {
+    N2_HIP(hipEventRecord(rb->done, s->control));
+    rb->busy = true;
+    rb->seq = ++s->est_seq;
+  }
+  eb.ids.clear();
+  eb.busy = false;
+}
+
+// Requests whose result copy has completed: free their arena range, done.
+int poll_results(n2wos_server* s, bool wait_oldest) {
   int finished = 0;
-  for (int q = 0; q < k; ++q) {
-    Request& r = *s->requests.at(round_done[q]);
-    r.rounds += 1;
-    r.wpp += r.round_walks;
-    r.est = std::sqrt(std::max(0.0, s->h_est[q]));
-    const int room = s->cfg.max_walks_per_point - r.wpp;
-    if (r.est <= r.target || room <= 0) {
-      const size_t p = static_cast<size_t>(r.n);
-      r.r_n.resize(p);
-      r.r_mean.resize(p);
-      r.r_m2.resize(p);
-      r.r_steps.resize(p);
-      r.r_forced.resize(p);
-      N2_HIP(hipMemcpyAsync(r.r_n.data(), r.d_n, 8 * p, hipMemcpyDeviceToHost, st));
-      N2_HIP(hipMemcpyAsync(r.r_mean.data(), r.d_mean, 8 * p, hipMemcpyDeviceToHost, st));
-      N2_HIP(hipMemcpyAsync(r.r_m2.data(), r.d_m2, 8 * p, hipMemcpyDeviceToHost, st));
-      N2_HIP(hipMemcpyAsync(r.r_steps.data(), r.d_steps, 8 * p, hipMemcpyDeviceToHost, st));
-      N2_HIP(hipMemcpyAsync(r.r_forced.data(), r.d_forced, 4 * p, hipMemcpyDeviceToHost, st));
-      N2_HIP(hipStreamSynchronize(st));
-      uint64_t tot = 0;
-      for (uint64_t v : r.r_steps) tot += v;
-      r.total_steps = tot;
-      s->steps += tot;
+  ResBatch* oldest = nullptr;
+  for (ResBatch& rb : s->res)
+    if (rb.busy && (!oldest || rb.seq < oldest->seq)) oldest = &rb;
+  if (wait_oldest && oldest) N2_HIP(hipEventSynchronize(oldest->done));
+  for (ResBatch& rb : s->res) {
+    if (!rb.busy) continue;
+    const hipError_t q = hipEventQuery(rb.done);
+    if (q == hipErrorNotReady) continue;
+    if (q != hipSuccess) N2_HIP(q);
+    for (int64_t id : rb.ids) {
+      Request& r = *s->requests.at(id);
       s->arena->release(r.off, r.bytes);
       r.off = -1;
       r.d_points = nullptr;
+      s->steps += r.total_steps;
       r.state = N2WOS_REQ_DONE;
       r.t_done = s->now();
       s->active.erase(std::find(s->active.begin(), s->active.end(), r.id));
       ++s->done;
       ++finished;
-    } else {
-      // walks still missing by the estimate (variance ~ 1 / n), 10 % margin
-      const double ratio = (r.est / r.target) * (r.est / r.target);
-      const double want = std::ceil(static_cast<double>(r.wpp) * (ratio - 1.0) * 1.1);
-      int64_t w = (static_cast<int64_t>(want) + kUnit - 1) / kUnit * kUnit;
-      w = std::max<int64_t>(kUnit, std::min<int64_t>({w, s->cfg.max_round_walks, room}));
-      r.round_walks = static_cast<int>(w);
-      r.cursor = 0;
     }
+    rb.ids.clear();
+    rb.busy = false;
   }
   return finished;
 }
 
-// One step: admit, keep every free slot busy, wait for the oldest launch to
-// complete, retire every completed launch, finish the rounds that ended, and
-// refill the freed slots before returning.
+EstBatch* oldest_est(n2wos_server* s) {
+  EstBatch* o = nullptr;
+  for (EstBatch& eb : s->est)
+    if (eb.busy && (!o || eb.seq < o->seq)) o = &eb;
+  return o;
+}
+
+// Queues the estimate of every request whose round has just been merged: a
+// kernel on the control stream, then an asynchronous copy of its decisions
+// to pinned memory and an event the host polls.  The requests schedule no
+// work until their decision is applied.
+int finish_rounds(n2wos_server* s, const std::vector<int64_t>& round_done) {
+  const int finished = 0;
+  size_t at = 0;
+  while (at < round_done.size()) {
+    EstBatch* eb = nullptr;
+    for (EstBatch& e : s->est)
+      if (!e.busy) {
+        eb = &e;
+        break;
+      }
+    if (!eb) {
+      // every batch in flight: wait for the oldest and apply it
+      EstBatch* o = oldest_est(s);
+      N2_HIP(hipEventSynchronize(o->done));
+      apply_estimates(s, *o);
+      continue;
+    }
+    const int k = static_cast<int>(std::min<size_t>(round_done.size() - at, kMaxPieces));
+    eb->ids.assign(round_done.begin() + static_cast<int64_t>(at), round_done.begin() + static_cast<int64_t>(at + k));
+    at += static_cast<size_t>(k);
+    for (int q = 0; q < k; ++q) {
+      Request& r = *s->requests.at(eb->ids[static_cast<size_t>(q)]);
+      r.rounds += 1;
+      r.wpp += r.round_walks;
+      const int room = s->cfg.max_walks_per_point - r.wpp;
+      eb->h_items[q] = EstItem{r.d_n, r.d_m2, r.d_steps, r.target, r.n, r.wpp, r.unit,
+                               room <= 0 ? 0 : std::min(s->cfg.max_round_walks, room / r.unit * r.unit)};
+    }
+    hipStream_t st = s->control;
+    N2_HIP(hipMemcpyAsync(eb->d_items, eb->h_items, sizeof(EstItem) * k, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_serve_estimate, dim3(k), dim3(kEstBlock), 0, st, eb->d_items, eb->d_out);
+    N2_HIP(hipGetLastError());
+    N2_HIP(hipMemcpyAsync(eb->h_out, eb->d_out, sizeof(EstOut) * k, hipMemcpyDeviceToHost, st));
+    N2_HIP(hipEventRecord(eb->done, st));
+    eb->busy = true;
+    eb->seq = ++s->est_seq;
+  }
+  return finished;
+}
+
+// Applies every estimate batch whose decisions have arrived.
+void poll_estimates(n2wos_server* s) {
+  for (EstBatch& eb : s->est) {
+    if (!eb.busy) continue;
+    const hipError_t q = hipEventQuery(eb.done);
+    if (q == hipSuccess) {
+      apply_estimates(s, eb);
+    } else if (q != hipErrorNotReady) {
+      N2_HIP(q);
+    }
+  }
+}
+
+// One step: admit, keep every free slot busy, wait for the oldest launch (or,
+// with none in flight, the oldest estimate batch), retire every completed
+// launch, queue the estimates of the rounds that ended, apply the decisions
+// that have arrived, and refill the freed slots before returning.
 void server_step(n2wos_server* s, int* n_done) {
   const double t_begin = s->now();
   server_admit(s);
@@ -521,8 +699,12 @@ void server_step(n2wos_server* s, int* n_done) {
     for (Slot& sl : m->slot)
       if (sl.busy && (!oldest || sl.seq < oldest->seq)) oldest = &sl;
   int finished = 0;
-  if (oldest) {
-    N2_HIP(hipEventSynchronize(oldest->done));
+  EstBatch* eo = oldest ? nullptr : oldest_est(s);
+  bool res_pending = false;
+  for (const ResBatch& rb : s->res) res_pending = res_pending || rb.busy;
+  if (oldest || eo || res_pending) {
+    // wait for the oldest launch, else the oldest estimate, else a result copy
+    if (oldest || eo) N2_HIP(hipEventSynchronize(oldest ? oldest->done : eo->done));
     std::vector<int64_t> round_done;
     for (auto& m : s->meshes)
       for (Slot& sl : m->slot) {
@@ -534,7 +716,9 @@ void server_step(n2wos_server* s, int* n_done) {
           N2_HIP(q);
         }
       }
-    finished = finish_rounds(s, round_done);
+    finished += finish_rounds(s, round_done);
+    poll_estimates(s);
+    finished += poll_results(s, !oldest && !eo);
     server_admit(s);
     launch_free_slots(s);
   }
@@ -543,13 +727,82 @@ void server_step(n2wos_server* s, int* n_done) {
   if (n_done) *n_done = finished;
 }
 
+// Device time covered by at least one launch, and from the first launch's
+// start to the last one's end, over the retired launches (seconds).
+void device_busy(const n2wos_server* s, double* busy, double* span) {
+  std::vector<std::pair<float, float>> iv = s->intervals;
+  std::sort(iv.begin(), iv.end());
+  double total = 0.0, cur_a = 0.0, cur_b = -1.0;
+  for (const auto& x : iv) {
+    if (x.first > cur_b) {
+      if (cur_b > cur_a) total += cur_b - cur_a;
+      cur_a = x.first;
+      cur_b = x.second;
+    } else if (x.second > cur_b) {
+      cur_b = x.second;
+    }
+  }
+  if (cur_b > cur_a) total += cur_b - cur_a;
+  *busy = total * 1e-3;
+  *span = iv.empty() ? 0.0 : (std::max_element(iv.begin(), iv.end(), [](const auto& a, const auto& b) {
+                                return a.second < b.second;
+                              })->second - iv.front().first) * 1e-3;
+}
+
 void free_slot(Slot& sl) {
   if (sl.stream) (void)hipStreamSynchronize(sl.stream);
   (void)hipFree(sl.buf);
   (void)hipFree(sl.d_pieces);
   (void)hipHostFree(sl.h_pieces);
   if (sl.done) (void)hipEventDestroy(sl.done);
+  if (sl.begin) (void)hipEventDestroy(sl.begin);
   if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  sl = Slot{};
+}
+
+// Allocates the mesh's batch slots for m.cap entries of m.unit walks each.
+void alloc_slots(n2wos_server* s, Mesh& m) {
+  const int64_t c = m.cap;
+  const int64_t ws_bytes = wos_workspace_bytes(m.cap, m.unit);
+  const int64_t sz[8] = {align256(12 * c), align256(8 * c), align256(8 * c), align256(8 * c),
+                         align256(8 * c), align256(8 * c), align256(4 * c), align256(ws_bytes)};
+  int64_t total = 0;
+  for (int64_t v : sz) total += v;
+  try {
+    for (Slot& sl : m.slot) {
+      if (hipMalloc(&sl.buf, total) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(N2WOS_ERR_NOMEM, "cannot allocate the mesh batch");
+      }
+      char* p = static_cast<char*>(sl.buf);
+      sl.bpts = reinterpret_cast<float*>(p);
+      p += sz[0];
+      sl.brng = reinterpret_cast<uint32_t*>(p);
+      p += sz[1];
+      sl.bn = reinterpret_cast<double*>(p);
+      p += sz[2];
+      sl.bmean = reinterpret_cast<double*>(p);
+      p += sz[3];
+      sl.bm2 = reinterpret_cast<double*>(p);
+      p += sz[4];
+      sl.bsteps = reinterpret_cast<uint64_t*>(p);
+      p += sz[5];
+      sl.bforced = reinterpret_cast<uint32_t*>(p);
+      p += sz[6];
+      sl.ws = p;
+      N2_HIP(hipMalloc(&sl.d_pieces, sizeof(ServePiece) * kMaxPieces));
+      N2_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_pieces), sizeof(ServePiece) * kMaxPieces));
+      N2_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+      N2_HIP(hipEventCreate(&sl.begin));
+      N2_HIP(hipEventCreate(&sl.done));
+    }
+  } catch (...) {
+    for (Slot& sl : m.slot) free_slot(sl);
+    throw;
+  }
+  s->fixed_bytes -= m.bytes;
+  m.bytes = kSlotsPerMesh * (total + static_cast<int64_t>(sizeof(ServePiece)) * kMaxPieces);
+  s->fixed_bytes += m.bytes;
 }
 
 }  // namespace
@@ -579,11 +832,18 @@ extern "C" int n2wos_server_create(const n2wos_server_config* cfg, n2wos_server*
     s->arena_base = static_cast<char*>(base);
     s->arena = std::make_unique<Arena>(cfg->hbm_budget_bytes);
     N2_HIP(hipStreamCreateWithFlags(&s->control, hipStreamNonBlocking));
-    N2_HIP(hipMalloc(&s->d_est_items, sizeof(EstItem) * kMaxPieces));
-    N2_HIP(hipMalloc(&s->d_est, sizeof(double) * kMaxPieces));
-    N2_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_est_items), sizeof(EstItem) * kMaxPieces));
-    N2_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_est), sizeof(double) * kMaxPieces));
-    s->fixed_bytes += sizeof(EstItem) * kMaxPieces + sizeof(double) * kMaxPieces;
+    for (EstBatch& eb : s->est) {
+      N2_HIP(hipMalloc(&eb.d_items, sizeof(EstItem) * kMaxPieces));
+      N2_HIP(hipMalloc(&eb.d_out, sizeof(EstOut) * kMaxPieces));
+      N2_HIP(hipHostMalloc(reinterpret_cast<void**>(&eb.h_items), sizeof(EstItem) * kMaxPieces));
+      N2_HIP(hipHostMalloc(reinterpret_cast<void**>(&eb.h_out), sizeof(EstOut) * kMaxPieces));
+      N2_HIP(hipEventCreateWithFlags(&eb.done, hipEventDisableTiming));
+      s->fixed_bytes += (sizeof(EstItem) + sizeof(EstOut)) * kMaxPieces;
+    }
+    // the origin of the device-time intervals of the launches
+    N2_HIP(hipEventCreate(&s->t_ref));
+    N2_HIP(hipEventRecord(s->t_ref, s->control));
+    N2_HIP(hipEventSynchronize(s->t_ref));
     *out = s.release();
     return N2WOS_OK;
   });
@@ -596,10 +856,18 @@ extern "C" int n2wos_server_destroy(n2wos_server* s) {
       for (Slot& sl : m->slot) free_slot(sl);
     if (s->control) (void)hipStreamSynchronize(s->control);
     (void)hipFree(s->arena_base);
-    (void)hipFree(s->d_est_items);
-    (void)hipFree(s->d_est);
-    (void)hipHostFree(s->h_est_items);
-    (void)hipHostFree(s->h_est);
+    for (auto& kv : s->requests)
+      if (kv.second->h_res) (void)hipHostFree(kv.second->h_res);
+    for (ResBatch& rb : s->res)
+      if (rb.done) (void)hipEventDestroy(rb.done);
+    for (EstBatch& eb : s->est) {
+      (void)hipFree(eb.d_items);
+      (void)hipFree(eb.d_out);
+      (void)hipHostFree(eb.h_items);
+      (void)hipHostFree(eb.h_out);
+      if (eb.done) (void)hipEventDestroy(eb.done);
+    }
+    if (s->t_ref) (void)hipEventDestroy(s->t_ref);
     if (s->control) (void)hipStreamDestroy(s->control);
     delete s;
     return N2WOS_OK;
@@ -621,47 +889,53 @@ extern "C" int n2wos_server_add_mesh(n2wos_server* s, const n2wos_bvh* bvh, cons
     m->wp.event_begin = nullptr;
     m->wp.event_end = nullptr;
     m->cap = static_cast<int>(s->cfg.tick_walks / kUnit);
-    const int64_t c = m->cap;
-    const int64_t ws_bytes = wos_workspace_bytes(m->cap, kUnit);
-    const int64_t sz[8] = {align256(12 * c), align256(8 * c), align256(8 * c), align256(8 * c),
-                           align256(8 * c), align256(8 * c), align256(4 * c), align256(ws_bytes)};
-    int64_t total = 0;
-    for (int64_t v : sz) total += v;
-    try {
-      for (Slot& sl : m->slot) {
-        if (hipMalloc(&sl.buf, total) != hipSuccess) {
-          (void)hipGetLastError();
-          throw Error(N2WOS_ERR_NOMEM, "cannot allocate the mesh batch");
-        }
-        char* p = static_cast<char*>(sl.buf);
-        sl.bpts = reinterpret_cast<float*>(p);
-        p += sz[0];
-        sl.brng = reinterpret_cast<uint32_t*>(p);
-        p += sz[1];
-        sl.bn = reinterpret_cast<double*>(p);
-        p += sz[2];
-        sl.bmean = reinterpret_cast<double*>(p);
-        p += sz[3];
-        sl.bm2 = reinterpret_cast<double*>(p);
-        p += sz[4];
-        sl.bsteps = reinterpret_cast<uint64_t*>(p);
-        p += sz[5];
-        sl.bforced = reinterpret_cast<uint32_t*>(p);
-        p += sz[6];
-        sl.ws = p;
-        N2_HIP(hipMalloc(&sl.d_pieces, sizeof(ServePiece) * kMaxPieces));
-        N2_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_pieces), sizeof(ServePiece) * kMaxPieces));
-        N2_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
-        N2_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
-      }
-    } catch (...) {
-      for (Slot& sl : m->slot) free_slot(sl);
-      throw;
-    }
-    m->bytes = kSlotsPerMesh * (total + static_cast<int64_t>(sizeof(ServePiece)) * kMaxPieces);
-    s->fixed_bytes += m->bytes;
+    m->unit = kUnit;
+    m->first_round = s->cfg.first_round_walks;
+    alloc_slots(s, *m);
     *mesh_id = static_cast<int>(s->meshes.size());
     s->meshes.push_back(std::move(m));
+    return N2WOS_OK;
+  });
+}
+
+extern "C" int n2wos_server_set_mesh_field(n2wos_server* s, int mesh_id, const n2wos_cv_field* field,
+                                           int unit_walks, int first_round_walks) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(s, N2WOS_ERR_NULL, "NULL server");
+    N2_REQUIRE(mesh_id >= 0 && mesh_id < static_cast<int>(s->meshes.size()), N2WOS_ERR_INVALID, "unknown mesh");
+    for (const auto& kv : s->requests)
+      N2_REQUIRE(kv.second->mesh != mesh_id, N2WOS_ERR_INVALID,
+                 "set a mesh's field before any request is submitted to it");
+    Mesh& m = *s->meshes[static_cast<size_t>(mesh_id)];
+    const bool cv = field && field->points != 0;
+    int unit = kUnit, first = s->cfg.first_round_walks;
+    if (cv) {
+      N2_REQUIRE(field->points == 2 || field->points == 6, N2WOS_ERR_INVALID, "field points must be 2 or 6");
+      N2_REQUIRE(field->d_values, N2WOS_ERR_NULL, "NULL field values");
+      N2_REQUIRE(field->res >= 1 && field->res <= 1024, N2WOS_ERR_INVALID, "field res must be in [1, 1024]");
+      for (int k = 0; k < 3; ++k)
+        N2_REQUIRE(std::isfinite(field->origin[k]) && std::isfinite(field->cell_inverse[k]) &&
+                       field->cell_inverse[k] > 0.0f,
+                   N2WOS_ERR_INVALID, "field box must be finite with positive cell_inverse");
+      N2_REQUIRE(std::isfinite(field->beta), N2WOS_ERR_INVALID, "field beta must be finite");
+      N2_REQUIRE(unit_walks >= 1 && unit_walks <= kUnit && (unit_walks & (unit_walks - 1)) == 0, N2WOS_ERR_INVALID,
+                 "unit_walks must be a power of two in [1, 64]");
+      N2_REQUIRE(first_round_walks >= unit_walks && first_round_walks % unit_walks == 0 &&
+                     first_round_walks <= s->cfg.max_round_walks && first_round_walks <= s->cfg.max_walks_per_point,
+                 N2WOS_ERR_INVALID, "first_round_walks must be a multiple of unit_walks within the round limits");
+      unit = unit_walks;
+      first = first_round_walks;
+    }
+    const int64_t cap = s->cfg.tick_walks / unit;
+    N2_REQUIRE(cap < (int64_t{1} << 30), N2WOS_ERR_INVALID, "tick_walks / unit_walks too large");
+    N2_HIP(hipDeviceSynchronize());
+    for (Slot& sl : m.slot) free_slot(sl);
+    m.cv = cv;
+    m.field = cv ? *field : n2wos_cv_field{};
+    m.unit = unit;
+    m.first_round = first;
+    m.cap = static_cast<int>(cap);
+    alloc_slots(s, m);
     return N2WOS_OK;
   });
 }
@@ -683,8 +957,10 @@ extern "C" int n2wos_server_submit(n2wos_server* s, int mesh_id, const float* ho
     r->n = n_points;
     r->target = target_rmse;
     r->rng_base = rng_base;
+    r->unit = s->meshes[static_cast<size_t>(mesh_id)]->unit;
     r->bytes = bytes;
     r->host_points.assign(host_points, host_points + 3 * static_cast<int64_t>(n_points));
+    N2_HIP(hipHostMalloc(reinterpret_cast<void**>(&r->h_res), result_bytes(n_points)));
     r->t_submit = s->now();
     *request_id = r->id;
     s->queue.push_back(r->id);
@@ -720,6 +996,7 @@ extern "C" int n2wos_server_request_info(const n2wos_server* s, int64_t request_
     info->t_submit = r.t_submit;
     info->t_admit = r.t_admit;
     info->t_done = r.t_done;
+    info->control_variate = s->meshes[static_cast<size_t>(r.mesh)]->cv ? 1 : 0;
     return N2WOS_OK;
   });
 }
@@ -732,11 +1009,15 @@ extern "C" int n2wos_server_take_result(n2wos_server* s, int64_t request_id, dou
     N2_REQUIRE(it != s->requests.end(), N2WOS_ERR_INVALID, "unknown request id");
     Request& r = *it->second;
     N2_REQUIRE(r.state == N2WOS_REQ_DONE, N2WOS_ERR_INVALID, "request has not finished");
-    if (n) std::copy(r.r_n.begin(), r.r_n.end(), n);
-    if (mean) std::copy(r.r_mean.begin(), r.r_mean.end(), mean);
-    if (m2) std::copy(r.r_m2.begin(), r.r_m2.end(), m2);
-    if (steps) std::copy(r.r_steps.begin(), r.r_steps.end(), steps);
-    if (forced) std::copy(r.r_forced.begin(), r.r_forced.end(), forced);
+    const size_t p = static_cast<size_t>(r.n);
+    const char* h = r.h_res;
+    if (n) std::memcpy(n, h, 8 * p);
+    if (mean) std::memcpy(mean, h + 8 * p, 8 * p);
+    if (m2) std::memcpy(m2, h + 16 * p, 8 * p);
+    if (steps) std::memcpy(steps, h + 24 * p, 8 * p);
+    if (forced) std::memcpy(forced, h + 32 * p, 4 * p);
+    (void)hipHostFree(r.h_res);
+    r.h_res = nullptr;
     s->requests.erase(it);
     return N2WOS_OK;
   });
@@ -754,6 +1035,8 @@ extern "C" int n2wos_server_get_stats(const n2wos_server* s, n2wos_server_stats*
     st->arena_used_bytes = s->arena->used();
     st->fixed_bytes = s->fixed_bytes;
     st->busy_seconds = s->busy;
+    st->launches = s->launches;
+    device_busy(s, &st->gpu_busy_seconds, &st->gpu_span_seconds);
     return N2WOS_OK;
   });
 }

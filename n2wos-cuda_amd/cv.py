@@ -158,3 +158,64 @@ class CvSolver:
         sel = k[None, :] <= wst[:, None].to(torch.int64)
         y = (wv[:, None] - rec.q)[sel]
         return rec.pos[sel].contiguous(), y.contiguous()
+
+
+def train_field(bvh, verts, boundary, params, points, cache_kw=None, label_stride: int = 2, path_len: int = 32,
+                rounds: int = 1, train_steps=(40, 40), batch: int = 1 << 15, res: int = 64, design: int = 6,
+                seed: int = 11, solver: "CvSolver | None" = None):
+    """The training pipeline of the control variate: labels from the path
+    records of one plain walk per labelled point, Adam steps, a bake of the
+    cache onto a res^3 grid over the padded mesh bounds, then `rounds`
+    bootstrap rounds (labels from control-variate walks with the current
+    field, far less noisy, and more Adam steps).  points: a CUDA (n, 3)
+    tensor of interior points (every label_stride-th is labelled).
+    Returns (field, cache, labels per round)."""
+    import torch
+
+    from . import cache as K
+    lo, hi = verts.min(0), verts.max(0)
+    cs = solver or CvSolver(bvh, boundary, params)
+    cache = K.Cache(K.CacheConfig.for_bounds(lo, hi, seed=seed, max_batch=batch, **(cache_kw or {})))
+    lp = points[::label_stride].contiguous() if label_stride > 1 else points
+    st = cs.solve(lp, 1, None, path_len=path_len, seed=seed + 101, stream_id=5)
+    x, y = CvSolver.path_labels(st)
+    cache.set_output_offset(float(y.mean().item()))
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    nlab = []
+    fld = None
+    for r in range(rounds + 1):
+        nlab.append(int(x.shape[0]))
+        for _ in range(train_steps[r]):
+            idx = torch.randint(0, x.shape[0], (batch,), device="cuda", generator=gen)
+            cache.train_step(x[idx].contiguous(), y[idx].contiguous())
+        fld = bake(cache, lo, hi, res=res, points=design)
+        if r < rounds:
+            st = cs.solve(lp, 1, fld, path_len=path_len, seed=seed + 200 + r, stream_id=6 + r)
+            x, y = CvSolver.path_labels(st)
+    return fld, cache, nlab
+
+
+def measure_gain(bvh, boundary, params, field: CvField, points, wpp: int = 16, seed: int = 3) -> dict:
+    """Plain walks against control-variate walks on the same points: mean
+    walk variance, walker time, and the gain (variance reduction / cost ratio
+    per walk) that decides whether a mesh is served with the field."""
+    import time
+
+    import torch
+
+    from .wos import Solver
+    plain, cs = Solver(bvh, boundary, params), CvSolver(bvh, boundary, params)
+    out = {}
+    for name, fn in (("plain", lambda: plain.solve(points, wpp, seed=seed)),
+                     ("cv", lambda: cs.solve(points, wpp, field, seed=seed))):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = fn()
+        torch.cuda.synchronize()
+        out[name + "_seconds"] = time.perf_counter() - t0
+        out[name + "_variance"] = float(st.variance().mean().item())
+    out["variance_reduction"] = out["plain_variance"] / max(out["cv_variance"], 1e-300)
+    out["cost_ratio"] = out["cv_seconds"] / out["plain_seconds"]
+    out["gain"] = out["variance_reduction"] / out["cost_ratio"]
+    return out

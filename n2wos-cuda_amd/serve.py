@@ -1,8 +1,11 @@
 """Request server over resident meshes (n2wos_server_* of the C ABI).
 
 The server itself -- admission by HBM budget, continuous batching of walker
-pools across requests, per-request rounds sized by the estimated RMSE -- is
-native code; this module wraps it and replays a request trace against it.
+pools across requests, per-request rounds sized on the device by the
+estimated RMSE, the learned cache as an along-walk control variate on meshes
+that carry a field -- is native code; this module wraps it, chooses per mesh
+between the control variate and plain walks by their measured gain, and
+replays a request trace against it.
 """
 from __future__ import annotations
 
@@ -26,13 +29,15 @@ class ServerConfigC(C.Structure):
 class RequestInfoC(C.Structure):
     _fields_ = [("state", C.c_int), ("mesh", C.c_int), ("n_points", C.c_int), ("rounds", C.c_int),
                 ("walks_per_point", C.c_int), ("estimated_rmse", C.c_double), ("walk_steps", C.c_uint64),
-                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+                ("control_variate", C.c_int)]
 
 
 class ServerStatsC(C.Structure):
     _fields_ = [("ticks", C.c_int64), ("requests_submitted", C.c_int64), ("requests_done", C.c_int64),
                 ("walks", C.c_int64), ("walk_steps", C.c_uint64), ("arena_bytes", C.c_int64),
-                ("arena_used_bytes", C.c_int64), ("fixed_bytes", C.c_int64), ("busy_seconds", C.c_double)]
+                ("arena_used_bytes", C.c_int64), ("fixed_bytes", C.c_int64), ("busy_seconds", C.c_double),
+                ("launches", C.c_int64), ("gpu_busy_seconds", C.c_double), ("gpu_span_seconds", C.c_double)]
 
 
 _SIGS = {
@@ -42,6 +47,7 @@ _SIGS = {
     "n2wos_server_submit": ([C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_uint32,
                              C.POINTER(C.c_int64)], C.c_int),
     "n2wos_server_step": ([C.c_void_p, C.POINTER(C.c_int)], C.c_int),
+    "n2wos_server_set_mesh_field": ([C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int], C.c_int),
     "n2wos_server_request_info": ([C.c_void_p, C.c_int64, C.POINTER(RequestInfoC)], C.c_int),
     "n2wos_server_take_result": ([C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p], C.c_int),
@@ -112,6 +118,15 @@ class Server:
         check(_L().n2wos_server_add_mesh(self._h, bvh.handle, C.byref(bc), C.byref(wp), C.byref(mid)))
         self._meshes.append((bvh, bc, wp))
         return mid.value
+
+    def set_mesh_field(self, mesh: int, field, unit_walks: int = 8, first_round_walks: int = 8) -> None:
+        """Serve the mesh with the along-walk control variate over `field` (a
+        cv.CvField, kept alive here) in rounds of unit_walks walks; None
+        restores plain walks.  Before any request to the mesh."""
+        fc = field.to_c() if field is not None else None
+        check(_L().n2wos_server_set_mesh_field(self._h, mesh, C.byref(fc) if fc is not None else None, unit_walks,
+                                               first_round_walks))
+        self._meshes.append((field, fc))
 
     def submit(self, mesh: int, points: np.ndarray, target_rmse: float = 1e-2, rng_base: int | None = None) -> int:
         pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
@@ -197,14 +212,47 @@ def replay(server: Server, arrivals, point_sets, meshes, target_rmse: float = 1e
     return ids, infos, lat, wall
 
 
+def mesh_fields(eng, meshes, log=None, n_train: int = 65536, n_probe: int = 4096, seed: int = 0, **train_kw):
+    """Per mesh: the control-variate field (cv.train_field on uniform interior
+    points) and its measured gain over plain walks on other interior points;
+    the policy serves the mesh with the field when the gain exceeds 1.
+    Returns one dict per mesh (field or None, gain measurement, seconds)."""
+    import time
+
+    import torch
+
+    from . import cv as CV, problems as P
+    out = []
+    for k, (name, v, bvh, bc, wp) in enumerate(meshes):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tr = torch.from_numpy(P.interior_points(bvh, v, n_train, seed=seed + 7000 + k)).cuda()
+        fld, _, nlab = CV.train_field(bvh, v, bc, wp, tr, **{"label_stride": 1, **train_kw})
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - t0
+        probe = torch.from_numpy(P.interior_points(bvh, v, n_probe, seed=seed + 8000 + k)).cuda()
+        g = CV.measure_gain(bvh, bc, wp, fld, probe)
+        use = g["gain"] > 1.0
+        out.append({"mesh": name, "field": fld if use else None, "use_control_variate": use, "train_seconds": secs,
+                    "labels": nlab, **g})
+        if log:
+            log(f"serving field {name}: variance reduction {g['variance_reduction']:.1f}x, cost {g['cost_ratio']:.2f}x,"
+                f" gain {g['gain']:.1f} -> {'control variate' if use else 'plain walks'}")
+    return out
+
+
 def serving_workload(eng, n_requests: int = 40, rate_per_s: float = 1.5, seed: int = 0, target_rmse: float = 1e-2,
-                     config: ServerConfig | None = None, min_points: int = 1000, max_points: int = 262144, log=None):
+                     config: ServerConfig | None = None, min_points: int = 1000, max_points: int = 262144, log=None,
+                     estimator: str = "auto", unit_walks: int = 8):
     """The serving benchmark: three resident meshes (UV sphere 64x128 and
     128x256 with r = 1 + 0.1 sin 6t sin 6p, and the 256x512 random-harmonic
     blob), each with its own 8-charge Dirichlet data; a seeded Poisson stream
-    of requests of log-uniform size; plain WoS at eps 1e-4 to the target RMSE.
-    Returns throughput, latency percentiles and the true RMSE of every
-    request against the FP64 solution."""
+    of requests of log-uniform size, WoS at eps 1e-4 to the target RMSE.
+    estimator "auto": each mesh gets a learned control-variate field when its
+    measured gain over plain walks exceeds 1 (plain walks otherwise); "plain":
+    plain walks everywhere.  Returns throughput, latency percentiles, the
+    device busy fraction and the true RMSE of every request against the FP64
+    solution."""
     from . import problems as P
     from .wos import WalkParams
     meshes = []
@@ -218,6 +266,12 @@ def serving_workload(eng, n_requests: int = 40, rate_per_s: float = 1.5, seed: i
     server = Server(config or ServerConfig(hbm_budget_bytes=1 << 30))
     for _, _, bvh, bc, wp in meshes:
         server.add_mesh(bvh, bc, wp)
+    policy = []
+    if estimator == "auto":
+        policy = mesh_fields(eng, meshes, log=log)
+        for k, pm in enumerate(policy):
+            if pm["field"] is not None:
+                server.set_mesh_field(k, pm["field"], unit_walks, unit_walks)
     if log:
         log(f"serving: {n_requests} requests, {int(sizes.sum())} points, rate {rate_per_s}/s")
     ids, infos, lat, wall = replay(server, arrivals, point_sets, mesh_of, target_rmse)
@@ -243,6 +297,13 @@ def serving_workload(eng, n_requests: int = 40, rate_per_s: float = 1.5, seed: i
         "latency_p99_s": float(np.percentile(lat, 99)),
         "latency_max_s": float(lat.max()),
         "ticks": st["ticks"],
+        "launches": st["launches"],
+        "gpu_busy_s": st["gpu_busy_seconds"],
+        "gpu_busy_fraction": st["gpu_busy_seconds"] / wall,
+        "latency_samples": int(len(lat)),
+        "estimator": estimator,
+        "mesh_policy": [{k: v for k, v in pm.items() if k != "field"} for pm in policy],
+        "control_variate_requests": int(sum(i.control_variate for i in infos)),
         "true_rmse_max": max(rmse),
         "true_rmse_mean": float(np.mean(rmse)),
         "walks_per_point_mean": float(np.mean([i.walks_per_point for i in infos])),

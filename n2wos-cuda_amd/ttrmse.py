@@ -105,9 +105,8 @@ def run_cv(bvh, verts, points, boundary, truth, params, plain_time, tau=1e-2, se
     pipeline is timed once; each estimate adds its own solve time."""
     import torch
 
-    from . import cache as K, cv as CV
+    from . import cv as CV
     tp = torch.from_numpy(points).cuda()
-    lo, hi = verts.min(0), verts.max(0)
     cs = CV.CvSolver(bvh, boundary, params)
     cs.solve(tp[:1024].contiguous(), 1, None, path_len=4)  # warm the extended walker
     out = []
@@ -118,23 +117,9 @@ def run_cv(bvh, verts, points, boundary, truth, params, plain_time, tau=1e-2, se
                "batch": su.batch, "field_res": su.res, "design_points": su.points}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        cache = K.Cache(K.CacheConfig.for_bounds(lo, hi, seed=seed, max_batch=su.batch, **su.cache_kw))
-        lp = tp[:: su.label_stride].contiguous()
-        st = cs.solve(lp, 1, None, path_len=su.path_len, seed=seed + 101, stream_id=5)
-        x, y = CV.CvSolver.path_labels(st)
-        cache.set_output_offset(float(y.mean().item()))
-        gen = torch.Generator(device="cuda").manual_seed(seed)
-        nlab = []
-        fld = None
-        for r in range(su.rounds + 1):
-            nlab.append(int(x.shape[0]))
-            for _ in range(su.train_steps[r]):
-                idx = torch.randint(0, x.shape[0], (su.batch,), device="cuda", generator=gen)
-                cache.train_step(x[idx].contiguous(), y[idx].contiguous())
-            fld = CV.bake(cache, lo, hi, res=su.res, points=su.points)
-            if r < su.rounds:
-                st = cs.solve(lp, 1, fld, path_len=su.path_len, seed=seed + 200 + r, stream_id=6 + r)
-                x, y = CV.CvSolver.path_labels(st)
+        fld, cache, nlab = CV.train_field(bvh, verts, boundary, params, tp, su.cache_kw, su.label_stride,
+                                          su.path_len, su.rounds, su.train_steps, su.batch, su.res, su.points,
+                                          seed, solver=cs)
         torch.cuda.synchronize()
         t_train = time.perf_counter() - t0
         res["train_seconds"] = t_train

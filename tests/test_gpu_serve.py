@@ -101,3 +101,59 @@ def test_budget_admission_order(eng, torch_gpu, meshes):
         s.submit(1, np.zeros((8000, 3), np.float32), 1e-2)
     assert e.value.code == _lib.ERR_NOMEM
     assert s.stats()["arena_used_bytes"] == 0
+
+
+def test_control_variate_mesh_alone_equals_mixed(eng, torch_gpu, meshes):
+    """A mesh served with a learned control-variate field (cache trained on
+    the mesh's own walks, baked, attached before any request): a request's
+    results are bit-identical alone and in a mix with plain-walk meshes,
+    agree with the direct control-variate solve of the same walks (same
+    values, merge order aside), meet the target against the FP64 solution,
+    and need far fewer walks than plain walks."""
+    from n2wos_amd import cv as CV, problems as P
+    torch = torch_gpu
+    name, v, bvh, bc, wp = meshes[1]
+    train = torch.from_numpy(P.interior_points(bvh, v, 16384, seed=81)).cuda()
+    print("case: serve control variate, field training", flush=True)
+    field, _, _ = CV.train_field(bvh, v, bc, wp, train, {"n_levels": 3, "base_resolution": 8, "growth": 2.0},
+                                 label_stride=1, rounds=1, train_steps=(40, 40))
+    sets = [P.interior_points(meshes[m][2], meshes[m][1], n, seed=90 + i)
+            for i, (m, n) in enumerate([(1, 2500), (0, 1200), (2, 900), (1, 700)])]
+    mesh_of = [1, 0, 2, 1]
+    bases = [0, 10_000, 20_000, 30_000]
+    target = 1e-2
+
+    print("case: serve control variate alone", flush=True)
+    s1 = make_server(eng, meshes, tick_walks=1 << 20)
+    s1.set_mesh_field(1, field, 8, 8)
+    a = s1.submit(1, sets[0], target, rng_base=bases[0])
+    s1.drain([a])
+    ia = s1.info(a)
+    alone = s1.take_result(a)
+    assert ia.control_variate == 1 and ia.estimated_rmse <= target
+    assert ia.walks_per_point < 64, ia.walks_per_point
+
+    print("case: serve control variate mixed with plain meshes", flush=True)
+    s2 = make_server(eng, meshes, tick_walks=1 << 14)
+    s2.set_mesh_field(1, field, 8, 8)
+    ids = [s2.submit(mesh_of[i], sets[i], 3e-2 if mesh_of[i] != 1 else target, rng_base=bases[i])
+           for i in range(4)]
+    s2.drain(ids)
+    assert [s2.info(i).control_variate for i in ids] == [1, 0, 0, 1]
+    assert s2.info(ids[0]).walks_per_point == ia.walks_per_point
+    mixed = [s2.take_result(i) for i in ids]
+    assert same(alone, mixed[0])
+    st = s2.stats()
+    assert st["requests_done"] == 4 and st["arena_used_bytes"] == 0
+    assert st["launches"] > 4 and 0 < st["gpu_busy_seconds"] <= st["gpu_span_seconds"] + 1e-9
+
+    print("case: served control variate vs direct solve", flush=True)
+    W_ = ia.walks_per_point
+    d = CV.CvSolver(bvh, bc, wp).solve(torch.from_numpy(sets[0]).cuda(), W_, field, point_offset=bases[0])
+    torch.cuda.synchronize()
+    assert np.array_equal(d.n.cpu().numpy(), alone.n)
+    assert np.array_equal(d.steps.cpu().numpy(), alone.steps)
+    assert np.allclose(d.mean.cpu().numpy(), alone.mean, rtol=1e-12, atol=1e-14)
+    truth = P.charges_solution_f64(bc.charges, sets[0])
+    rmse = float(np.sqrt(np.mean((alone.mean - truth) ** 2)))
+    assert rmse <= 1.25 * target, rmse
