@@ -15,7 +15,11 @@ Prints ONE JSON line.  `value` is walk-steps/s of the whole job.  `roofline`
 prices the walker kernel's algorithmic bytes (node lines x 112 B + triangle
 records x 40 B + 12 B per walk start + 24 B per work unit, visit counts from a
 counting run with the same seeds) against MI355X HBM peak, with the kernel's
-duration measured by HIP events on its own stream.  `cpu_baseline` times the
+duration measured by HIP events on its own stream.  The roof that binds the
+walker is VALU issue: `roofline.valu` gives its counters (VALU instructions
+per walk step, busy and lane-utilisation fractions) and `roofline.traffic` the
+HBM bytes per launch, both read from profiles/ only when the profile was
+taken of this build's walker sources (`source_sha256`), else null.  `cpu_baseline` times the
 oracle's CPU walker (same walks, same Philox streams) on a sample of the
 points.  Extra figures sit under "extra":
   closest_point_problem1   65,536 uniform queries on the 64 x 128 sphere
@@ -78,16 +82,63 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip problem 3, time-to-RMSE and serving")
-    ap.add_argument("--serve-requests", type=int, default=12)
-    ap.add_argument("--serve-rate", type=float, default=0.3, help="request arrivals per second")
+    ap.add_argument("--serve-requests", type=int, default=240)
+    ap.add_argument("--serve-rate", type=float, default=24.0, help="request arrivals per second")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run a 1 GiB device copy (counter calibration) and one walker solve, then exit; "
                          "used under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--source-hash", action="store_true", help="print the walker sources' hash and exit")
     ap.add_argument("--section", choices=["walker", "cache", "closest_stress", "poisson"], default=None,
                     help="run one section's GPU work only (one headline solve; the cache's training steps and "
                          "inference; the closest-point stress sets; problem 3), print a short JSON line and "
                          "exit: the workload of the counter profiles")
     return ap.parse_args()
+
+
+WALKER_SOURCES = ("walker_kernel.hpp", "traverse.hpp", "geom.hpp", "cvfield.hpp", "wave.hpp", "bvh.hpp", "wos.hpp",
+                  "wos.hip", "bvh_build.cpp")
+
+
+def walker_source_hash() -> str:
+    """SHA-256 of the walker's sources and build flags: a counter profile
+    applies to the build whose hash it records."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in WALKER_SOURCES:
+        with open(os.path.join(ROOT, "n2wos-cuda_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    with open(os.path.join(ROOT, "n2wos-cuda_amd", "Makefile"), "rb") as fh:
+        h.update(fh.read())
+    return h.hexdigest()
+
+
+def profile_for_build(name: str):
+    """profiles/<name> when it was measured on this build's walker, else None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        doc = json.load(f)
+    return doc if doc.get("source_sha256") == walker_source_hash() else None
+
+
+def host_cpu() -> dict:
+    """Host CPU facts for the CPU baseline: online CPUs, this process's
+    affinity, the model string."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "affinity": aff, "model": model}
 
 
 def cpu_threads() -> int:
@@ -289,6 +340,9 @@ def cache_rates(eng, torch) -> dict:
 
 def main():
     args = parse()
+    if args.source_hash:
+        print(walker_source_hash())
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
@@ -363,11 +417,14 @@ def main():
     avg_ms = sum(kernel_ms) / len(kernel_ms)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
 
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "wos_walker_mesh2_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    tprof = profile_for_build("wos_walker_mesh2_traffic.json")
+    traffic = tprof.get("hbm_bytes_per_launch") if tprof else None
+    vprof = profile_for_build("wos_walker_mesh2_valu.json")
+    valu = None
+    if vprof:
+        valu = {k: vprof.get(k) for k in ("valu_insts_per_step", "valu_busy", "valu_utilization", "waves_per_cu",
+                                          "kernel_ms", "source")}
+        valu["frac"] = vprof.get("valu_busy")
 
     truth = P.charges_solution_f64(bc.charges, pts)
     mean = stats[0].mean.cpu().numpy()
@@ -421,6 +478,9 @@ def main():
             "achievable_GBps": achievable,
             "frac_of_achievable": achieved / achievable,
             "traffic": traffic,
+            "binding_roof": "valu_issue",
+            "valu": valu,
+            "source_sha256": walker_source_hash(),
             "kernel": "k_wos_persistent",
             "kernel_ms_avg": avg_ms,
             "algorithmic_bytes_per_launch": bytes_launch,
@@ -455,8 +515,8 @@ def main():
                                               config=S.ServerConfig(hbm_budget_bytes=1 << 30),
                                               log=log)
         sv = extra["serving"]
-        log(f"serving: p50 {sv['latency_p50_s']:.2f} s p99 {sv['latency_p99_s']:.2f} s, "
-            f"{sv['walk_steps_per_s'] / 1e9:.3f} G steps/s")
+        log(f"serving: p50 {sv['latency_p50_s']:.3f} s p99 {sv['latency_p99_s']:.3f} s over {sv['latency_samples']} "
+            f"requests, device busy {sv['gpu_busy_fraction']:.2f}, {sv['points_per_s']:.0f} points/s")
 
     if not args.no_cpu:
         import oracle
@@ -479,6 +539,7 @@ def main():
         _, st_cpu, _, _ = b2.walks(sample, args.wpp, max_steps=256, eps=1e-4, step_scale=0.999, bmode=bc.mode,
                                    bparams=bc.oracle_params(), seed=params.seed, point_ids=ids, nthreads=threads)
         dt = time.perf_counter() - t0
+        hc = host_cpu()
         line["cpu_baseline"] = {
             "value": float(st_cpu.sum() / dt),
             "unit": "walk-steps/s",
@@ -486,6 +547,12 @@ def main():
             "kind": "port",
             "sample": f"{len(sample)} of {n} grid points (every {stride}th) x {args.wpp} walks, same walks as the "
                       f"GPU run, {dt:.1f} s on {threads} host threads",
+            "threads": threads,
+            "nproc": hc["nproc"],
+            "affinity": hc["affinity"],
+            "cpu_model": hc["model"],
+            "threads_note": "all the host CPUs one GPU is allotted (16 per GPU on the GPU box), "
+                            "capped at this process's affinity",
         }
     print(json.dumps(line))
 

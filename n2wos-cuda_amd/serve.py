@@ -212,6 +212,12 @@ def replay(server: Server, arrivals, point_sets, meshes, target_rmse: float = 1e
     return ids, infos, lat, wall
 
 
+# the control-variate cache of the serving meshes: a 3-level dense grid
+# (N_min 8, growth 2) with the 4 x 64 MLP, the shape that fits single-walk
+# labels best on the benchmark problem (finer grids overfit them)
+SERVE_FIELD_CACHE = {"n_levels": 3, "base_resolution": 8, "growth": 2.0}
+
+
 def mesh_fields(eng, meshes, log=None, n_train: int = 65536, n_probe: int = 4096, seed: int = 0, **train_kw):
     """Per mesh: the control-variate field (cv.train_field on uniform interior
     points) and its measured gain over plain walks on other interior points;
@@ -227,7 +233,8 @@ def mesh_fields(eng, meshes, log=None, n_train: int = 65536, n_probe: int = 4096
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         tr = torch.from_numpy(P.interior_points(bvh, v, n_train, seed=seed + 7000 + k)).cuda()
-        fld, _, nlab = CV.train_field(bvh, v, bc, wp, tr, **{"label_stride": 1, **train_kw})
+        fld, _, nlab = CV.train_field(bvh, v, bc, wp, tr, **{"label_stride": 1, "cache_kw": SERVE_FIELD_CACHE,
+                                                             **train_kw})
         torch.cuda.synchronize()
         secs = time.perf_counter() - t0
         probe = torch.from_numpy(P.interior_points(bvh, v, n_probe, seed=seed + 8000 + k)).cuda()
