@@ -35,3 +35,35 @@ def test_probe_cli_problem1(eng, torch_gpu, tmp_path):
     assert v["bit_exact_matches"] == 4096 and v["triangle_id_mismatches"] == 0
     assert v["cuda_stack_overflow_count"] == 0
     assert len(j["timing"]["kernel_ms"]) == 3 and j["timing"]["median_mqueries_per_second"] > 0
+
+
+@pytest.mark.parametrize("mode", ["wos_like_prefix", "near_boundary_shell"])
+def test_geometry_backend_sweep(eng, torch_gpu, tmp_path, mode):
+    """The builder / leaf-size sweep probe: every backend (median and SAH
+    builders x two leaf sizes, both query kernels) validates bit for bit
+    against the host brute force, the v2 JSON carries the reference's keys and
+    backend_policy labels, and the recommendation names the fastest passing
+    backend."""
+    exe = os.path.join(os.path.dirname(eng.LIB_PATH), "..", "bin", "n2wos_probe_geometry_backends")
+    out = str(tmp_path / "sweep.json")
+    args = [exe, "--mesh", "sin6", "--bumpy-stacks", "64", "--bumpy-slices", "128", "--bumpy-amplitude", "0.1",
+            "--queries", "32768", "--validate", "1024", "--repeat", "2", "--query-mode", mode,
+            "--cubql-build-methods", "spatial_median,sah", "--leaf-sizes", "2,8", "--query-kernels", "quad,lane",
+            "--output", out]
+    print("case: geometry backend sweep %s" % mode, flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    j = json.load(open(out))
+    assert j["schema"] == "n2wos_geometry_backend_probe_v2"
+    for k in ("backend_policy", "options", "mesh_stats", "build_features", "recommendation", "backends"):
+        assert k in j, k
+    assert j["backend_policy"]["production_requires_gpu_resident_geometry_query"] is True
+    b = {k: v for k, v in j["backends"].items() if v.get("built")}
+    assert len(b) == 8 and not j["backends"]["cubql_cuda"]["enabled"]
+    for name, v in b.items():
+        val = v["validation"]
+        assert v["validation_passed"] and v["overflow_count"] == 0, name
+        assert val["checked"] == 1024 and val["bit_exact_matches"] == 1024, name
+        assert val["max_backend_distance_delta"] == 0.0, name
+    best = min(b, key=lambda k: b[k]["timing"]["median_us_per_query"])
+    assert j["recommendation"]["selected_production_candidate"] == best
