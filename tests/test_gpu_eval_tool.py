@@ -80,3 +80,39 @@ def test_slice_mask_equals_reference_run(eng, torch_gpu):
     got = mask.reshape(h, w)[::-1].astype(bool)  # image rows: v descending
     assert int(got.sum()) == int(g["inside_pixels"]) == len(pts)
     assert np.array_equal(got, want)
+
+
+WTOOL = os.path.join(ROOT, "n2wos-cuda_amd", "tools", "n2wos_eval_wavefront_wos.py")
+
+
+@pytest.mark.parametrize("method", ["pure_wos", "oracle_2lmc", "oracle_residual"])
+def test_eval_wavefront_methods(tmp_path, torch_gpu, method):
+    """The wavefront evaluation tool at x0 on mesh 1, boundary x^2 - y^2:
+    the v1 JSON keys, every estimate within 4.5 standard errors of the exact
+    value (the residual's of zero), and the analytic-oracle coarse level far
+    less noisy than plain walks."""
+    out = tmp_path / "wf.json"
+    args = [sys.executable, WTOOL, "--mesh", "mesh1", "--method", method, "--samples", "16384",
+            "--residual-samples", "4096", "--depth-m", "4", "--epsilon", "1e-3", "--x0", "0.2,-0.1,0.05",
+            "--output", str(out)]
+    print("case: eval wavefront %s" % method, flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    doc = json.loads(out.read_text())
+    assert doc["schema"] == "n2wos_wavefront_wos_eval_v1"
+    for k in ("implementation_mode", "options", "mesh_stats", "bvh_stats", "target", "runs", "estimator"):
+        assert k in doc, k
+    assert doc["implementation_mode"]["cache_backend"] == "analytic_oracle_debug_only"
+    est = doc["estimator"]
+    exact = 0.2 ** 2 - 0.1 ** 2
+    if method == "oracle_residual":
+        run = doc["runs"]["oracle_residual"]
+        assert abs(run["mean"]) < 4.5 * run["stderr"] + 1e-12
+    else:
+        assert abs(est["exact"] - exact) < 1e-12
+        assert abs(est["error"]) < 4.5 * est["stderr"], est
+    if method == "oracle_2lmc":
+        coarse, resid = doc["runs"]["coarse"], doc["runs"]["residual"]
+        assert coarse["samples"] == 16384 and resid["samples"] == 4096
+        assert coarse["mean_steps"] <= 4.0
+        assert abs(est["estimator_variance"] - coarse["estimator_variance"] - resid["estimator_variance"]) < 1e-15
