@@ -323,9 +323,9 @@ int n2wos_nc_combine(int n_points, int walks_per_point, int mode, const float* d
                      double* d_m2, void* stream);
 
 /* ------------------------------------------------------------ learned cache */
-/* Multiresolution hash grid + ReLU MLP (2 n_levels inputs, 4 hidden layers of
- * 64, 1 output), trained with Adam on the MSE loss.  Replaces the
- * tiny-cuda-nn NetworkWithInputEncoding / Trainer / inference calls of
+/* Multiresolution hash grid + ReLU MLP (2 n_levels inputs, n_hidden_layers
+ * hidden layers of n_neurons, 1 output), trained with Adam on the MSE loss.
+ * Replaces the tiny-cuda-nn NetworkWithInputEncoding / Trainer / inference calls of
  * src/tools/eval_tcnn_nc_wos.cu:747-769 (HashGrid + FullyFusedMLP, L2 loss,
  * Adam).  MLP operands are FP16 on MFMA with FP32 accumulation and a static
  * loss scale; tables and master weights are FP32.  World points map to the
@@ -343,8 +343,14 @@ typedef struct {
   float loss_scale;      /* FP16 gradient scale */
   uint64_t seed;         /* parameter initialisation */
   int max_batch;         /* largest training batch (0: 262144); its activation
-                            store (1,090 B per sample) is allocated here */
+                            store (2 (33 + 2 L H) B per sample) is allocated here */
+  int n_neurons;         /* hidden width H: 16, 32 or 64 (0: 64) */
+  int n_hidden_layers;   /* hidden layers L: 1 .. 4 (0: 4) */
 } n2wos_cache_config;
+/* MLP weights, FP32 row-major, concatenated: A_1 [H x 32] (input features
+ * 2l, 2l+1 of level l; levels >= n_levels read zero), A_2 .. A_L [H x H],
+ * A_out [16 x H] (row 0 is the output, rows 1..15 are zero):
+ * 32 H + (L - 1) H^2 + 16 H floats (15,360 for 64 x 4). */
 int n2wos_cache_create(const n2wos_cache_config* config, n2wos_cache** out);
 int n2wos_cache_destroy(n2wos_cache* cache);
 /* A constant added to the network's output, in inference and in the

@@ -8,14 +8,22 @@ import numpy as np
 
 from ._lib import check, lib
 
-N_WEIGHTS = 15360
+N_WEIGHTS = 15360  # the default 4 x 64 MLP
+
+
+def n_weights(n_neurons: int = 64, n_hidden_layers: int = 4) -> int:
+    """Weight-buffer length of the MLP: A1 [H x 32], the hidden layers
+    [H x H], the output block [16 x H] (row 0 live)."""
+    h = n_neurons
+    return 32 * h + (n_hidden_layers - 1) * h * h + 16 * h
 
 
 class CacheConfigC(C.Structure):
     _fields_ = [("n_levels", C.c_int), ("log2_table_size", C.c_int), ("base_resolution", C.c_int),
                 ("growth", C.c_float), ("origin", C.c_float * 3), ("extent", C.c_float * 3),
                 ("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
-                ("loss_scale", C.c_float), ("seed", C.c_uint64), ("max_batch", C.c_int)]
+                ("loss_scale", C.c_float), ("seed", C.c_uint64), ("max_batch", C.c_int), ("n_neurons", C.c_int),
+                ("n_hidden_layers", C.c_int)]
 
 
 @dataclass
@@ -35,6 +43,8 @@ class CacheConfig:
     loss_scale: float = 128.0
     seed: int = 1
     max_batch: int = 1 << 18  # largest training batch (activation store allocated at creation)
+    n_neurons: int = 64       # MLP width: 16, 32 or 64
+    n_hidden_layers: int = 4  # MLP depth: 1 to 4
 
     @classmethod
     def for_bounds(cls, lo, hi, pad: float = 0.05, **kw):
@@ -53,6 +63,7 @@ class CacheConfig:
             c.extent[a] = self.extent[a]
         c.learning_rate, c.beta1, c.beta2, c.epsilon = self.learning_rate, self.beta1, self.beta2, self.epsilon
         c.loss_scale, c.seed, c.max_batch = self.loss_scale, self.seed, self.max_batch
+        c.n_neurons, c.n_hidden_layers = self.n_neurons, self.n_hidden_layers
         return c
 
 
@@ -120,7 +131,7 @@ class Cache:
     def get_params(self):
         lay = self.layout()
         grid = np.zeros(2 * lay["total_entries"], np.float32)
-        w = np.zeros(N_WEIGHTS, np.float32)
+        w = np.zeros(n_weights(self.config.n_neurons, self.config.n_hidden_layers), np.float32)
         check(lib().n2wos_cache_get_params(self.handle, grid.ctypes.data, w.ctypes.data))
         return grid, w
 
@@ -133,7 +144,7 @@ class Cache:
     def gradients(self, points, targets):
         lay = self.layout()
         gg = np.zeros(2 * lay["total_entries"], np.float32)
-        wg = np.zeros(N_WEIGHTS, np.float32)
+        wg = np.zeros(n_weights(self.config.n_neurons, self.config.n_hidden_layers), np.float32)
         loss = np.zeros(1, np.float32)
         check(lib().n2wos_cache_gradients(self.handle, points.data_ptr(), targets.data_ptr(), points.shape[0],
                                           gg.ctypes.data, wg.ctypes.data, loss.ctypes.data))

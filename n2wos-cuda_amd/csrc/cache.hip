@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "api_internal.hpp"
@@ -125,66 +126,56 @@ __device__ __forceinline__ h4_t frag_trans(const _Float16* m, int cols, int r, i
   return a;
 }
 
-// Forward of one 16-sample tile; returns the output accumulator (row 0 of the
-// padded output layer, held in element 0 of lanes with g == 0) and the
-// hidden activations.  Every MFMA here and below runs with the whole wave
-// active: the instruction reads all 64 lanes' operands.
+// Forward of one 16-sample tile through the MLP with L hidden layers of width
+// H (T = H / 16 tiles per layer); returns the output accumulator (row 0 of
+// the padded output layer, held in element 0 of lanes with g == 0) and the
+// hidden activations h[layer][tile].  Every MFMA here and below runs with the
+// whole wave active: the instruction reads all 64 lanes' operands.
+template <int H, int L>
 __device__ __forceinline__ f4_t mlp_forward(const _Float16* __restrict__ w, const h4_t xb[2], int i, int g,
-                                            h4_t h1[4], h4_t h2[4], h4_t h3[4], h4_t h4[4]) {
-  const _Float16* a1 = w;
-  const _Float16* a2 = w + kCacheOffA2;
-  const _Float16* a3 = w + kCacheOffA3;
-  const _Float16* a4 = w + kCacheOffA4;
-  const _Float16* a5 = w + kCacheOffA5;
+                                            h4_t (&h)[L][H / 16]) {
+  constexpr int T = H / 16;
 #pragma unroll
-  for (int o = 0; o < 4; ++o) {
+  for (int o = 0; o < T; ++o) {
     f4_t acc = zero4();
 #pragma unroll
-    for (int t = 0; t < 2; ++t) acc = mfma16(frag_rows(a1, kCacheIn, 16 * o + i, 16 * t + 4 * g), xb[t], acc);
-    h1[o] = relu_h4(acc);
+    for (int t = 0; t < 2; ++t) acc = mfma16(frag_rows(w, kCacheIn, 16 * o + i, 16 * t + 4 * g), xb[t], acc);
+    h[0][o] = relu_h4(acc);
   }
 #pragma unroll
-  for (int o = 0; o < 4; ++o) {
-    f4_t acc = zero4();
+  for (int k = 1; k < L; ++k) {
+    const _Float16* a = w + cache_off_hidden(H, k);
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc = mfma16(frag_rows(a2, kCacheHidden, 16 * o + i, 16 * t + 4 * g), h1[t], acc);
-    h2[o] = relu_h4(acc);
+    for (int o = 0; o < T; ++o) {
+      f4_t acc = zero4();
+#pragma unroll
+      for (int t = 0; t < T; ++t) acc = mfma16(frag_rows(a, H, 16 * o + i, 16 * t + 4 * g), h[k - 1][t], acc);
+      h[k][o] = relu_h4(acc);
+    }
   }
-#pragma unroll
-  for (int o = 0; o < 4; ++o) {
-    f4_t acc = zero4();
-#pragma unroll
-    for (int t = 0; t < 4; ++t) acc = mfma16(frag_rows(a3, kCacheHidden, 16 * o + i, 16 * t + 4 * g), h2[t], acc);
-    h3[o] = relu_h4(acc);
-  }
-#pragma unroll
-  for (int o = 0; o < 4; ++o) {
-    f4_t acc = zero4();
-#pragma unroll
-    for (int t = 0; t < 4; ++t) acc = mfma16(frag_rows(a4, kCacheHidden, 16 * o + i, 16 * t + 4 * g), h3[t], acc);
-    h4[o] = relu_h4(acc);
-  }
+  const _Float16* ao = w + cache_off_out(H, L);
   f4_t out = zero4();
 #pragma unroll
-  for (int t = 0; t < 4; ++t) out = mfma16(frag_rows(a5, kCacheHidden, i, 16 * t + 4 * g), h4[t], out);
+  for (int t = 0; t < T; ++t) out = mfma16(frag_rows(ao, H, i, 16 * t + 4 * g), h[L - 1][t], out);
   return out;
 }
 
 constexpr int kCacheBlock = 256;
 
-__device__ __forceinline__ void stage_weights(_Float16* lds, const _Float16* __restrict__ w) {
-  for (int k = threadIdx.x; k < kCacheWeights / 4; k += blockDim.x)
+__device__ __forceinline__ void stage_weights(_Float16* lds, const _Float16* __restrict__ w, int count) {
+  for (int k = threadIdx.x; k < count / 4; k += blockDim.x)
     reinterpret_cast<h4_t*>(lds)[k] = reinterpret_cast<const h4_t*>(w)[k];
   __syncthreads();
 }
 
 // Inference: out[s] = C(points[s]) (+ the output offset, when set) for s < n.
+template <int H, int L>
 __global__ void __launch_bounds__(kCacheBlock) k_cache_infer(GridLayout gl, const float* __restrict__ grid,
                                                             const _Float16* __restrict__ wh,
                                                             const float* __restrict__ points, int n,
                                                             float offset, float* __restrict__ out) {
-  __shared__ _Float16 w[kCacheWeights];
-  stage_weights(w, wh);
+  __shared__ _Float16 w[cache_weights(H, L)];
+  stage_weights(w, wh, cache_weights(H, L));
   const int lane = static_cast<int>(threadIdx.x & 63);
   const int i = lane & 15, g = lane >> 4;
   const int waves = static_cast<int>(gridDim.x * blockDim.x / 64);
@@ -196,21 +187,22 @@ __global__ void __launch_bounds__(kCacheBlock) k_cache_infer(GridLayout gl, cons
     if (s < n) to_unit(gl, points[3 * s], points[3 * s + 1], points[3 * s + 2], ux, uy, uz);
     h4_t xb[2];
     encode_fragments(gl, grid, ux, uy, uz, g, xb);
-    h4_t h1[4], h2[4], h3[4], h4[4];
-    const f4_t y = mlp_forward(w, xb, i, g, h1, h2, h3, h4);
+    h4_t h[L][H / 16];
+    const f4_t y = mlp_forward<H, L>(w, xb, i, g, h);
     if (g == 0 && s < n) out[s] = offset != 0.0f ? y[0] + offset : y[0];
   }
 }
 
 // Training pass 1: forward + backward of every 16-sample tile.  Stores the
-// layer inputs X, H1..H4 and the pre-activation gradients dZ1..dZ4, dY (FP16,
-// [feature][batch]) for the weight-gradient GEMMs, scatters the input-feature
-// gradients into the grid gradient, and adds the loss to *loss_sum.
+// layer inputs X, H1..H_L and the pre-activation gradients dZ1..dZ_L, dY
+// (FP16, [feature][batch]) for the weight-gradient GEMMs, scatters the
+// input-feature gradients into the grid gradient, and adds the loss to
+// *loss_sum.
 struct TrainStore {
   _Float16* x;    // [32][B]
-  _Float16* h;    // [4][64][B]  H1..H4
-  _Float16* dz;   // [4][64][B]  dZ1..dZ4
-  _Float16* dy;   // [B]         dL/dy * loss_scale
+  _Float16* h;    // [L][H][B]  H1..H_L
+  _Float16* dz;   // [L][H][B]  dZ1..dZ_L
+  _Float16* dy;   // [B]        dL/dy * loss_scale
   int batch;      // B (multiple of 16)
 };
 
@@ -221,8 +213,7 @@ __device__ __forceinline__ void store_frag(_Float16* base, int batch, int feat0,
   base[static_cast<size_t>(feat0 + 3) * batch + s] = v[3];
 }
 
-__device__ __forceinline__ h4_t mask_relu(f4_t d, h4_t h, float& dummy) {
-  (void)dummy;
+__device__ __forceinline__ h4_t mask_relu(f4_t d, h4_t h) {
   h4_t r;
   r[0] = static_cast<_Float16>(static_cast<float>(h[0]) > 0.0f ? d[0] : 0.0f);
   r[1] = static_cast<_Float16>(static_cast<float>(h[1]) > 0.0f ? d[1] : 0.0f);
@@ -240,6 +231,7 @@ __device__ __forceinline__ uint32_t xcd_id() { return __builtin_amdgcn_s_getreg(
 // that fewer atomics meet on each address (copy_mask 7), large ones one copy
 // (copy_mask 0).  Every copy is updated with device-scope atomics, so the sum
 // is right whichever XCD a block runs on.
+template <int H, int L>
 __global__ void __launch_bounds__(kCacheBlock) k_cache_train_pass1(GridLayout gl, const float* __restrict__ grid,
                                                                   const _Float16* __restrict__ wh,
                                                                   const float* __restrict__ points,
@@ -248,19 +240,16 @@ __global__ void __launch_bounds__(kCacheBlock) k_cache_train_pass1(GridLayout gl
                                                                   float* __restrict__ grid_grad_copies,
                                                                   float* __restrict__ loss_sum, uint32_t copy_mask,
                                                                   size_t gstride) {
+  constexpr int T = H / 16;
   float* __restrict__ grid_grad = grid_grad_copies + (xcd_id() & copy_mask) * gstride;
-  __shared__ _Float16 w[kCacheWeights];
-  stage_weights(w, wh);
+  __shared__ _Float16 w[cache_weights(H, L)];
+  stage_weights(w, wh, cache_weights(H, L));
   const int lane = static_cast<int>(threadIdx.x & 63);
   const int i = lane & 15, g = lane >> 4;
   const int waves = static_cast<int>(gridDim.x * blockDim.x / 64);
   const int wave = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / 64);
   const int tiles = st.batch / 16;
-  const _Float16* a1 = w;
-  const _Float16* a2 = w + kCacheOffA2;
-  const _Float16* a3 = w + kCacheOffA3;
-  const _Float16* a4 = w + kCacheOffA4;
-  const _Float16* a5 = w + kCacheOffA5;
+  const size_t HB = static_cast<size_t>(H) * st.batch;
   float loss = 0.0f;
   for (int tile = wave; tile < tiles; tile += waves) {
     const int s = tile * 16 + i;
@@ -269,8 +258,8 @@ __global__ void __launch_bounds__(kCacheBlock) k_cache_train_pass1(GridLayout gl
     if (live) to_unit(gl, points[3 * s], points[3 * s + 1], points[3 * s + 2], ux, uy, uz);
     h4_t xb[2];
     encode_fragments(gl, grid, ux, uy, uz, g, xb);
-    h4_t h1[4], h2[4], h3[4], h4[4];
-    const f4_t y = mlp_forward(w, xb, i, g, h1, h2, h3, h4);
+    h4_t h[L][T];
+    const f4_t y = mlp_forward<H, L>(w, xb, i, g, h);
     // dL/dy for L = mean (y - t)^2, scaled; broadcast from lane (i, 0) to (i, g)
     float dy = 0.0f;
     if (g == 0 && live) {
@@ -281,48 +270,37 @@ __global__ void __launch_bounds__(kCacheBlock) k_cache_train_pass1(GridLayout gl
     dy = __shfl(dy, i);
     // stores for the weight gradients
     for (int t = 0; t < 2; ++t) store_frag(st.x, st.batch, 16 * t + 4 * g, s, xb[t]);
-    for (int o = 0; o < 4; ++o) {
-      store_frag(st.h + 0 * 64 * static_cast<size_t>(st.batch), st.batch, 16 * o + 4 * g, s, h1[o]);
-      store_frag(st.h + 1 * 64 * static_cast<size_t>(st.batch), st.batch, 16 * o + 4 * g, s, h2[o]);
-      store_frag(st.h + 2 * 64 * static_cast<size_t>(st.batch), st.batch, 16 * o + 4 * g, s, h3[o]);
-      store_frag(st.h + 3 * 64 * static_cast<size_t>(st.batch), st.batch, 16 * o + 4 * g, s, h4[o]);
-    }
+#pragma unroll
+    for (int k = 0; k < L; ++k)
+#pragma unroll
+      for (int o = 0; o < T; ++o) store_frag(st.h + k * HB, st.batch, 16 * o + 4 * g, s, h[k][o]);
     if (g == 0) st.dy[s] = static_cast<_Float16>(dy);
-    // output layer: dH4 = A5^T dY, dY = [dy; 0; ...] (16 x 16)
+    // output layer: dH_L = A_out^T dY, dY = [dy; 0; ...] (16 x 16)
     h4_t bdy;
     bdy[0] = static_cast<_Float16>(g == 0 ? dy : 0.0f);
     bdy[1] = bdy[2] = bdy[3] = static_cast<_Float16>(0.0f);
-    float dmy = 0.0f;
-    h4_t dz4[4], dz3[4], dz2[4], dz1[4];
+    h4_t dz[T], dzp[T];
+    const _Float16* ao = w + cache_off_out(H, L);
 #pragma unroll
-    for (int o = 0; o < 4; ++o) {
-      f4_t acc = mfma16(frag_trans(a5, kCacheHidden, 16 * o + i, 4 * g), bdy, zero4());
-      dz4[o] = mask_relu(acc, h4[o], dmy);
-      store_frag(st.dz + 3 * 64 * static_cast<size_t>(st.batch), st.batch, 16 * o + 4 * g, s, dz4[o]);
+    for (int o = 0; o < T; ++o) {
+      const f4_t acc = mfma16(frag_trans(ao, H, 16 * o + i, 4 * g), bdy, zero4());
+      dz[o] = mask_relu(acc, h[L - 1][o]);
+      store_frag(st.dz + (L - 1) * HB, st.batch, 16 * o + 4 * g, s, dz[o]);
     }
+    // hidden layers, top down: dZ_k = relu'(H_k) (A_{k+1}^T dZ_{k+1})
 #pragma unroll
-    for (int o = 0; o < 4; ++o) {
-      f4_t acc = zero4();
+    for (int k = L - 1; k >= 1; --k) {
+      const _Float16* a = w + cache_off_hidden(H, k);
 #pragma unroll
-      for (int t = 0; t < 4; ++t) acc = mfma16(frag_trans(a4, kCacheHidden, 16 * o + i, 16 * t + 4 * g), dz4[t], acc);
-      dz3[o] = mask_relu(acc, h3[o], dmy);
-      store_frag(st.dz + 2 * 64 * static_cast<size_t>(st.batch), st.batch, 16 * o + 4 * g, s, dz3[o]);
-    }
+      for (int o = 0; o < T; ++o) {
+        f4_t acc = zero4();
 #pragma unroll
-    for (int o = 0; o < 4; ++o) {
-      f4_t acc = zero4();
+        for (int t = 0; t < T; ++t) acc = mfma16(frag_trans(a, H, 16 * o + i, 16 * t + 4 * g), dz[t], acc);
+        dzp[o] = mask_relu(acc, h[k - 1][o]);
+        store_frag(st.dz + (k - 1) * HB, st.batch, 16 * o + 4 * g, s, dzp[o]);
+      }
 #pragma unroll
-      for (int t = 0; t < 4; ++t) acc = mfma16(frag_trans(a3, kCacheHidden, 16 * o + i, 16 * t + 4 * g), dz3[t], acc);
-      dz2[o] = mask_relu(acc, h2[o], dmy);
-      store_frag(st.dz + 1 * 64 * static_cast<size_t>(st.batch), st.batch, 16 * o + 4 * g, s, dz2[o]);
-    }
-#pragma unroll
-    for (int o = 0; o < 4; ++o) {
-      f4_t acc = zero4();
-#pragma unroll
-      for (int t = 0; t < 4; ++t) acc = mfma16(frag_trans(a2, kCacheHidden, 16 * o + i, 16 * t + 4 * g), dz2[t], acc);
-      dz1[o] = mask_relu(acc, h1[o], dmy);
-      store_frag(st.dz + 0 * 64 * static_cast<size_t>(st.batch), st.batch, 16 * o + 4 * g, s, dz1[o]);
+      for (int o = 0; o < T; ++o) dz[o] = dzp[o];
     }
     // input-feature gradients dX = A1^T dZ1 (32 x 16), computed by the whole
     // wave (an MFMA reads every lane's operands), then scattered to the grid
@@ -331,12 +309,12 @@ __global__ void __launch_bounds__(kCacheBlock) k_cache_train_pass1(GridLayout gl
     for (int t = 0; t < 2; ++t) {
       f4_t acc = zero4();
 #pragma unroll
-      for (int k = 0; k < 4; ++k) acc = mfma16(frag_trans(a1, kCacheIn, 16 * t + i, 16 * k + 4 * g), dz1[k], acc);
+      for (int k = 0; k < T; ++k) acc = mfma16(frag_trans(w, kCacheIn, 16 * t + i, 16 * k + 4 * g), dz[k], acc);
       if (live) {
-        for (int h = 0; h < 2; ++h) {
-          const int l = 8 * t + 2 * g + h;
+        for (int hh = 0; hh < 2; ++hh) {
+          const int l = 8 * t + 2 * g + hh;
           if (l >= gl.n_levels) continue;
-          const float g0 = acc[2 * h], g1 = acc[2 * h + 1];
+          const float g0 = acc[2 * hh], g1 = acc[2 * hh + 1];
           if (g0 == 0.0f && g1 == 0.0f) continue;
           uint32_t idx[8];
           float wc[8];
@@ -354,11 +332,12 @@ __global__ void __launch_bounds__(kCacheBlock) k_cache_train_pass1(GridLayout gl
   if (lane == 0) atomicAdd(loss_sum, loss);
 }
 
-// Training pass 2: weight gradients dA_l = dZ_l H_{l-1}^T over the batch
-// (K = samples) on MFMA.  blockIdx.y = layer (0..4); each wave reduces a
-// K-chunk into ROWS_T x COLS_T accumulator tiles and adds them to the FP32
-// gradient with atomics.  ROWS_T = 0 marks the output layer, whose dZ is the
-// single row dY (rows 1..15 of the padded tile are zero).
+// Training pass 2: weight gradients dA = dZ H_prev^T over the batch (K =
+// samples) on MFMA.  blockIdx.y = layer (0 = input layer, 1 .. L-1 hidden,
+// L = output); each wave reduces a K-chunk into ROWS_T x COLS_T accumulator
+// tiles and adds them to the FP32 gradient with atomics.  ROWS_T = 0 marks
+// the output layer, whose dZ is the single row dY (rows 1..15 of the padded
+// tile are zero).
 template <int ROWS_T, int COLS_T>
 __device__ __forceinline__ void wgrad_layer(const _Float16* __restrict__ dz, const _Float16* __restrict__ hp,
                                             size_t B, int k0, int k1, int i, int g, int cols, float* __restrict__ wg) {
@@ -384,7 +363,7 @@ __device__ __forceinline__ void wgrad_layer(const _Float16* __restrict__ dz, con
         af = load_h4(dz + static_cast<size_t>(16 * r + i) * B + k + 4 * g);
       }
 #pragma unroll
-      for (int c = 0; c < COLS_T; ++c) acc[R * 0 + r * COLS_T + c] = mfma16(af, bf[c], acc[r * COLS_T + c]);
+      for (int c = 0; c < COLS_T; ++c) acc[r * COLS_T + c] = mfma16(af, bf[c], acc[r * COLS_T + c]);
     }
   }
   // acc[r * COLS_T + c] holds D[4g + e][i] of tile (r, c): row 16r + 4g + e, col 16c + i
@@ -400,7 +379,9 @@ __device__ __forceinline__ void wgrad_layer(const _Float16* __restrict__ dz, con
       }
 }
 
+template <int H, int L>
 __global__ void __launch_bounds__(kCacheBlock) k_cache_wgrad(TrainStore st, int chunk, float* __restrict__ w_grad) {
+  constexpr int T = H / 16;
   const int layer = static_cast<int>(blockIdx.y);
   const int lane = static_cast<int>(threadIdx.x & 63);
   const int i = lane & 15, g = lane >> 4;
@@ -409,13 +390,14 @@ __global__ void __launch_bounds__(kCacheBlock) k_cache_wgrad(TrainStore st, int 
   if (k0 >= st.batch) return;
   const int k1 = min(st.batch, k0 + chunk);
   const size_t B = static_cast<size_t>(st.batch);
+  const size_t HB = static_cast<size_t>(H) * B;
   if (layer == 0) {
-    wgrad_layer<4, 2>(st.dz, st.x, B, k0, k1, i, g, kCacheIn, w_grad);
-  } else if (layer < 4) {
-    wgrad_layer<4, 4>(st.dz + layer * 64 * B, st.h + (layer - 1) * 64 * B, B, k0, k1, i, g, kCacheHidden,
-                      w_grad + kCacheW1 + (layer - 1) * kCacheWh);
+    wgrad_layer<T, 2>(st.dz, st.x, B, k0, k1, i, g, kCacheIn, w_grad);
+  } else if (layer < L) {
+    wgrad_layer<T, T>(st.dz + layer * HB, st.h + (layer - 1) * HB, B, k0, k1, i, g, H,
+                      w_grad + cache_off_hidden(H, layer));
   } else {
-    wgrad_layer<0, 4>(st.dy, st.h + 3 * 64 * B, B, k0, k1, i, g, kCacheHidden, w_grad + kCacheOffA5);
+    wgrad_layer<0, T>(st.dy, st.h + (L - 1) * HB, B, k0, k1, i, g, H, w_grad + cache_off_out(H, L));
   }
 }
 
@@ -489,6 +471,9 @@ struct n2wos_cache {
   int64_t step = 0;
   float lr = 1e-2f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-15f, loss_scale = 128.0f;
   float out_offset = 0.0f;  // added to the network's output (inference and loss)
+  int hidden = 64;            // hidden width H (16, 32 or 64)
+  int layers = 4;             // hidden layers L (1 .. 4)
+  int n_w = n2::cache_weights(64, 4);
 };
 
 using namespace n2;
@@ -496,6 +481,65 @@ using namespace n2;
 namespace {
 
 size_t grid_floats(const n2wos_cache* c) { return 2 * static_cast<size_t>(c->layout.total_entries); }
+
+// Calls f(std::integral_constant<int, H>, std::integral_constant<int, L>) for
+// the cache's MLP shape: the kernels are compiled for H in {16, 32, 64} and
+// L in {1, 2, 3, 4}.
+template <int H, typename F>
+void with_depth(int layers, F&& f) {
+  switch (layers) {
+    case 1: f(std::integral_constant<int, H>{}, std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, H>{}, std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, H>{}, std::integral_constant<int, 3>{}); break;
+    default: f(std::integral_constant<int, H>{}, std::integral_constant<int, 4>{}); break;
+  }
+}
+
+template <typename F>
+void with_shape(const n2wos_cache* c, F&& f) {
+  switch (c->hidden) {
+    case 16: with_depth<16>(c->layers, f); break;
+    case 32: with_depth<32>(c->layers, f); break;
+    default: with_depth<64>(c->layers, f); break;
+  }
+}
+
+// Activation store of a batch of B (multiple of 16) samples: X [32][B], H and
+// dZ [L][H][B] each, dY [B], all FP16.
+size_t store_halves(int hidden, int layers, size_t batch) {
+  return batch * (static_cast<size_t>(kCacheIn) + 2 * static_cast<size_t>(layers) * hidden + 1);
+}
+
+TrainStore make_store(const n2wos_cache* c, int batch) {
+  TrainStore st;
+  st.batch = batch;
+  st.x = c->store;
+  st.h = st.x + static_cast<size_t>(kCacheIn) * batch;
+  st.dz = st.h + static_cast<size_t>(c->layers) * c->hidden * batch;
+  st.dy = st.dz + static_cast<size_t>(c->layers) * c->hidden * batch;
+  return st;
+}
+
+void launch_pass1(const n2wos_cache* c, const float* d_points, const float* d_targets, int n, float grad_scale,
+                  const TrainStore& st, hipStream_t s) {
+  const int tiles = st.batch / 16;
+  const int blocks = std::max(1, std::min(2048, (tiles + 3) / 4));
+  with_shape(c, [&](auto HC, auto LC) {
+    hipLaunchKernelGGL((k_cache_train_pass1<decltype(HC)::value, decltype(LC)::value>), dim3(blocks),
+                       dim3(kCacheBlock), 0, s, c->layout, c->grid, c->wh, d_points, d_targets, n, c->out_offset,
+                       grad_scale, st, c->grid_grad, c->loss, static_cast<uint32_t>(c->grad_copies - 1),
+                       grid_floats(c));
+  });
+}
+
+void launch_wgrad(const n2wos_cache* c, const TrainStore& st, hipStream_t s) {
+  const int chunk = 1024;
+  const int waves = (st.batch + chunk - 1) / chunk;
+  with_shape(c, [&](auto HC, auto LC) {
+    hipLaunchKernelGGL((k_cache_wgrad<decltype(HC)::value, decltype(LC)::value>),
+                       dim3((waves + 3) / 4, decltype(LC)::value + 1), dim3(kCacheBlock), 0, s, st, chunk, c->w_grad);
+  });
+}
 
 void free_cache(n2wos_cache* c) {
   for (void* p : {static_cast<void*>(c->grid), static_cast<void*>(c->grid_grad), static_cast<void*>(c->grid_m),
@@ -517,7 +561,14 @@ extern "C" int n2wos_cache_create(const n2wos_cache_config* cfg, n2wos_cache** o
                "log2_table_size must be in [8, 24]");
     N2_REQUIRE(cfg->base_resolution >= 1 && cfg->growth >= 1.0f, N2WOS_ERR_INVALID, "bad resolution parameters");
     for (int a = 0; a < 3; ++a) N2_REQUIRE(cfg->extent[a] > 0.0f, N2WOS_ERR_INVALID, "extent must be positive");
+    const int hidden = cfg->n_neurons > 0 ? cfg->n_neurons : kCacheMaxHidden;
+    const int layers = cfg->n_hidden_layers > 0 ? cfg->n_hidden_layers : kCacheMaxLayers;
+    N2_REQUIRE(hidden == 16 || hidden == 32 || hidden == 64, N2WOS_ERR_INVALID, "n_neurons must be 16, 32 or 64");
+    N2_REQUIRE(layers >= 1 && layers <= kCacheMaxLayers, N2WOS_ERR_INVALID, "n_hidden_layers must be in [1, 4]");
     auto* c = new n2wos_cache();
+    c->hidden = hidden;
+    c->layers = layers;
+    c->n_w = cache_weights(hidden, layers);
     try {
       GridLayout& L = c->layout;
       L.n_levels = cfg->n_levels;
@@ -554,38 +605,41 @@ extern "C" int n2wos_cache_create(const n2wos_cache_config* cfg, n2wos_cache** o
       N2_HIP(hipMalloc(&c->grid_grad, gf * 4 * c->grad_copies));
       N2_HIP(hipMalloc(&c->grid_m, gf * 4));
       N2_HIP(hipMalloc(&c->grid_v, gf * 4));
-      N2_HIP(hipMalloc(&c->w, kCacheWeights * 4));
-      N2_HIP(hipMalloc(&c->w_grad, kCacheWeights * 4));
-      N2_HIP(hipMalloc(&c->w_m, kCacheWeights * 4));
-      N2_HIP(hipMalloc(&c->w_v, kCacheWeights * 4));
-      N2_HIP(hipMalloc(&c->wh, kCacheWeights * 2));
+      const size_t nw = static_cast<size_t>(c->n_w);
+      N2_HIP(hipMalloc(&c->w, nw * 4));
+      N2_HIP(hipMalloc(&c->w_grad, nw * 4));
+      N2_HIP(hipMalloc(&c->w_m, nw * 4));
+      N2_HIP(hipMalloc(&c->w_v, nw * 4));
+      N2_HIP(hipMalloc(&c->wh, nw * 2));
       N2_HIP(hipMalloc(&c->bad, 64));
       N2_HIP(hipMalloc(&c->loss, 64));
       // the activation store of the largest batch: training allocates nothing
       c->store_batch = (max_batch + 15) / 16 * 16;
-      N2_HIP(hipMalloc(&c->store, static_cast<size_t>(c->store_batch) * (kCacheIn + 8 * kCacheHidden + 1) * 2));
+      N2_HIP(hipMalloc(&c->store, store_halves(hidden, layers, static_cast<size_t>(c->store_batch)) * 2));
       N2_HIP(hipMemset(c->grid_grad, 0, gf * 4 * c->grad_copies));
       N2_HIP(hipMemset(c->grid_m, 0, gf * 4));
       N2_HIP(hipMemset(c->grid_v, 0, gf * 4));
-      N2_HIP(hipMemset(c->w_grad, 0, kCacheWeights * 4));
-      N2_HIP(hipMemset(c->w_m, 0, kCacheWeights * 4));
-      N2_HIP(hipMemset(c->w_v, 0, kCacheWeights * 4));
+      N2_HIP(hipMemset(c->w_grad, 0, nw * 4));
+      N2_HIP(hipMemset(c->w_m, 0, nw * 4));
+      N2_HIP(hipMemset(c->w_v, 0, nw * 4));
       N2_HIP(hipMemset(c->bad, 0, 64));
       // grid: uniform +-1e-4; MLP: Xavier-uniform per layer (Philox, seeded)
       hipLaunchKernelGGL(k_init_uniform, dim3(1024), dim3(256), 0, nullptr, c->grid, gf, -1e-4f, 1e-4f,
                          static_cast<uint32_t>(cfg->seed), 0u);
-      const int fan[5][3] = {{kCacheIn, kCacheHidden, 0}, {kCacheHidden, kCacheHidden, kCacheOffA2},
-                             {kCacheHidden, kCacheHidden, kCacheOffA3}, {kCacheHidden, kCacheHidden, kCacheOffA4},
-                             {kCacheHidden, 1, kCacheOffA5}};
-      for (int l = 0; l < 5; ++l) {
-        const float a = std::sqrt(6.0f / static_cast<float>(fan[l][0] + fan[l][1]));
-        const size_t cnt = static_cast<size_t>(fan[l][0]) * fan[l][1];
-        hipLaunchKernelGGL(k_init_uniform, dim3(8), dim3(256), 0, nullptr, c->w + fan[l][2], cnt, -a, a,
+      // layer k (0 = input, 1 .. L-1 hidden, L = output): fan in, fan out,
+      // offset; Philox stream tag k + 1
+      for (int l = 0; l <= layers; ++l) {
+        const int fin = l == 0 ? kCacheIn : hidden;
+        const int fout = l == layers ? 1 : hidden;
+        const int off = l == 0 ? 0 : (l == layers ? cache_off_out(hidden, layers) : cache_off_hidden(hidden, l));
+        const float a = std::sqrt(6.0f / static_cast<float>(fin + fout));
+        const size_t cnt = static_cast<size_t>(fin) * fout;
+        hipLaunchKernelGGL(k_init_uniform, dim3(8), dim3(256), 0, nullptr, c->w + off, cnt, -a, a,
                            static_cast<uint32_t>(cfg->seed), static_cast<uint32_t>(l + 1));
       }
-      N2_HIP(hipMemset(c->w + kCacheOffA5 + kCacheHidden, 0, (kCacheW5 - kCacheHidden) * 4));
-      hipLaunchKernelGGL(k_to_half, dim3((kCacheWeights + 255) / 256), dim3(256), 0, nullptr, c->w, c->wh,
-                         kCacheWeights);
+      // rows 1..15 of the padded output layer stay zero
+      N2_HIP(hipMemset(c->w + cache_off_out(hidden, layers) + hidden, 0, static_cast<size_t>(15 * hidden) * 4));
+      hipLaunchKernelGGL(k_to_half, dim3((c->n_w + 255) / 256), dim3(256), 0, nullptr, c->w, c->wh, c->n_w);
       N2_HIP(hipGetLastError());
       N2_HIP(hipDeviceSynchronize());
     } catch (...) {
@@ -612,8 +666,11 @@ extern "C" int n2wos_cache_infer(const n2wos_cache* c, const float* d_points, in
     N2_REQUIRE(d_points && d_out, N2WOS_ERR_NULL, "NULL device pointer");
     const int tiles = (n + 15) / 16;
     const int blocks = std::max(1, std::min(4096, (tiles + 3) / 4));
-    hipLaunchKernelGGL(k_cache_infer, dim3(blocks), dim3(kCacheBlock), 0, static_cast<hipStream_t>(stream),
-                       c->layout, c->grid, c->wh, d_points, n, c->out_offset, d_out);
+    with_shape(c, [&](auto HC, auto LC) {
+      hipLaunchKernelGGL((k_cache_infer<decltype(HC)::value, decltype(LC)::value>), dim3(blocks), dim3(kCacheBlock),
+                         0, static_cast<hipStream_t>(stream), c->layout, c->grid, c->wh, d_points, n, c->out_offset,
+                         d_out);
+    });
     N2_HIP(hipGetLastError());
     return N2WOS_OK;
   });
@@ -630,25 +687,12 @@ extern "C" int n2wos_cache_train_step(n2wos_cache* c, const float* d_points, con
     N2_REQUIRE(batch <= c->store_batch, N2WOS_ERR_INVALID,
                "batch of " + std::to_string(n) + " exceeds the cache's max_batch (" +
                    std::to_string(c->store_batch) + ")");
-    TrainStore st;
-    st.batch = batch;
-    st.x = c->store;
-    st.h = st.x + static_cast<size_t>(kCacheIn) * batch;
-    st.dz = st.h + static_cast<size_t>(4 * kCacheHidden) * batch;
-    st.dy = st.dz + static_cast<size_t>(4 * kCacheHidden) * batch;
+    const TrainStore st = make_store(c, batch);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, c->bad);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, reinterpret_cast<int*>(c->loss));
-    const float grad_scale = c->loss_scale / static_cast<float>(n);
-    const int tiles = batch / 16;
-    const int blocks = std::max(1, std::min(2048, (tiles + 3) / 4));
-    hipLaunchKernelGGL(k_cache_train_pass1, dim3(blocks), dim3(kCacheBlock), 0, s, c->layout, c->grid, c->wh,
-                       d_points, d_targets, n, c->out_offset, grad_scale, st, c->grid_grad, c->loss,
-                       static_cast<uint32_t>(c->grad_copies - 1), grid_floats(c));
-    const int chunk = 1024;
-    const int waves = (batch + chunk - 1) / chunk;
-    hipLaunchKernelGGL(k_cache_wgrad, dim3((waves + 3) / 4, 5), dim3(kCacheBlock), 0, s, st, chunk, c->w_grad);
-    hipLaunchKernelGGL(k_check_finite, dim3(64), dim3(256), 0, s, c->w_grad, static_cast<size_t>(kCacheWeights),
-                       c->bad);
+    launch_pass1(c, d_points, d_targets, n, c->loss_scale / static_cast<float>(n), st, s);
+    launch_wgrad(c, st, s);
+    hipLaunchKernelGGL(k_check_finite, dim3(64), dim3(256), 0, s, c->w_grad, static_cast<size_t>(c->n_w), c->bad);
     hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, s, c->grid_grad,
                        grid_floats(c) * static_cast<size_t>(c->grad_copies), c->bad);
     c->step += 1;
@@ -658,10 +702,10 @@ extern "C" int n2wos_cache_train_step(n2wos_cache* c, const float* d_points, con
     const float eps_t = static_cast<float>(c->eps * std::sqrt(bc2));
     const float inv = 1.0f / c->loss_scale;
     hipLaunchKernelGGL(k_adam, dim3(64), dim3(256), 0, s, c->w, c->w_grad, c->w_m, c->w_v,
-                       static_cast<size_t>(kCacheWeights), 1, lr_t, c->beta1, c->beta2, eps_t, inv, c->bad);
+                       static_cast<size_t>(c->n_w), 1, lr_t, c->beta1, c->beta2, eps_t, inv, c->bad);
     hipLaunchKernelGGL(k_adam, dim3(2048), dim3(256), 0, s, c->grid, c->grid_grad, c->grid_m, c->grid_v,
                        grid_floats(c), c->grad_copies, lr_t, c->beta1, c->beta2, eps_t, inv, c->bad);
-    hipLaunchKernelGGL(k_to_half, dim3((kCacheWeights + 255) / 256), dim3(256), 0, s, c->w, c->wh, kCacheWeights);
+    hipLaunchKernelGGL(k_to_half, dim3((c->n_w + 255) / 256), dim3(256), 0, s, c->w, c->wh, c->n_w);
     N2_HIP(hipGetLastError());
     if (d_loss_out) N2_HIP(hipMemcpyAsync(d_loss_out, c->loss, 4, hipMemcpyDeviceToDevice, s));
     return N2WOS_OK;
@@ -687,12 +731,13 @@ extern "C" int n2wos_cache_copy_params(n2wos_cache* dst, const n2wos_cache* src,
     const GridLayout& b = src->layout;
     bool same = a.n_levels == b.n_levels && a.table_size == b.table_size && a.total_entries == b.total_entries;
     for (int l = 0; same && l < kCacheLevels; ++l) same = a.offset[l] == b.offset[l] && a.res[l] == b.res[l];
+    same = same && dst->hidden == src->hidden && dst->layers == src->layers;
     N2_REQUIRE(same, N2WOS_ERR_INVALID, "caches have different layouts");
     if (dst == src) return N2WOS_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
     N2_HIP(hipMemcpyAsync(dst->grid, src->grid, grid_floats(src) * 4, hipMemcpyDeviceToDevice, s));
-    N2_HIP(hipMemcpyAsync(dst->w, src->w, kCacheWeights * 4, hipMemcpyDeviceToDevice, s));
-    N2_HIP(hipMemcpyAsync(dst->wh, src->wh, kCacheWeights * 2, hipMemcpyDeviceToDevice, s));
+    N2_HIP(hipMemcpyAsync(dst->w, src->w, static_cast<size_t>(src->n_w) * 4, hipMemcpyDeviceToDevice, s));
+    N2_HIP(hipMemcpyAsync(dst->wh, src->wh, static_cast<size_t>(src->n_w) * 2, hipMemcpyDeviceToDevice, s));
     dst->out_offset = src->out_offset;
     return N2WOS_OK;
   });
@@ -717,13 +762,14 @@ extern "C" int n2wos_cache_layout(const n2wos_cache* c, int* n_levels, uint32_t*
 }
 
 // Host copies of (or host uploads to) the FP32 parameters: grid [2 * entries],
-// MLP weights [15360] (layout in the header); either pointer may be NULL.
+// MLP weights [32H + (L-1)H^2 + 16H] (layout in the header); either pointer
+// may be NULL.
 extern "C" int n2wos_cache_get_params(const n2wos_cache* c, float* grid, float* weights) {
   return guarded([&]() -> int {
     N2_REQUIRE(c != nullptr, N2WOS_ERR_NULL, "NULL cache");
     N2_HIP(hipDeviceSynchronize());
     if (grid) N2_HIP(hipMemcpy(grid, c->grid, grid_floats(c) * 4, hipMemcpyDeviceToHost));
-    if (weights) N2_HIP(hipMemcpy(weights, c->w, kCacheWeights * 4, hipMemcpyDeviceToHost));
+    if (weights) N2_HIP(hipMemcpy(weights, c->w, static_cast<size_t>(c->n_w) * 4, hipMemcpyDeviceToHost));
     return N2WOS_OK;
   });
 }
@@ -734,9 +780,8 @@ extern "C" int n2wos_cache_set_params(n2wos_cache* c, const float* grid, const f
     N2_HIP(hipDeviceSynchronize());
     if (grid) N2_HIP(hipMemcpy(c->grid, grid, grid_floats(c) * 4, hipMemcpyHostToDevice));
     if (weights) {
-      N2_HIP(hipMemcpy(c->w, weights, kCacheWeights * 4, hipMemcpyHostToDevice));
-      hipLaunchKernelGGL(k_to_half, dim3((kCacheWeights + 255) / 256), dim3(256), 0, nullptr, c->w, c->wh,
-                         kCacheWeights);
+      N2_HIP(hipMemcpy(c->w, weights, static_cast<size_t>(c->n_w) * 4, hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(k_to_half, dim3((c->n_w + 255) / 256), dim3(256), 0, nullptr, c->w, c->wh, c->n_w);
       N2_HIP(hipGetLastError());
     }
     N2_HIP(hipDeviceSynchronize());
@@ -754,20 +799,10 @@ extern "C" int n2wos_cache_gradients(n2wos_cache* c, const float* d_points, cons
     N2_REQUIRE(n >= 1 && d_points && d_targets, N2WOS_ERR_INVALID, "bad batch");
     const int batch = (n + 15) / 16 * 16;
     N2_REQUIRE(batch <= c->store_batch, N2WOS_ERR_INVALID, "batch exceeds the cache's max_batch");
-    TrainStore st;
-    st.batch = batch;
-    st.x = c->store;
-    st.h = st.x + static_cast<size_t>(kCacheIn) * batch;
-    st.dz = st.h + static_cast<size_t>(4 * kCacheHidden) * batch;
-    st.dy = st.dz + static_cast<size_t>(4 * kCacheHidden) * batch;
+    const TrainStore st = make_store(c, batch);
     N2_HIP(hipMemset(c->loss, 0, 4));
-    const int tiles = batch / 16;
-    hipLaunchKernelGGL(k_cache_train_pass1, dim3(std::max(1, std::min(2048, (tiles + 3) / 4))), dim3(kCacheBlock), 0,
-                       nullptr, c->layout, c->grid, c->wh, d_points, d_targets, n, c->out_offset,
-                       c->loss_scale / static_cast<float>(n), st, c->grid_grad, c->loss,
-                       static_cast<uint32_t>(c->grad_copies - 1), grid_floats(c));
-    const int waves = (batch + 1023) / 1024;
-    hipLaunchKernelGGL(k_cache_wgrad, dim3((waves + 3) / 4, 5), dim3(kCacheBlock), 0, nullptr, st, 1024, c->w_grad);
+    launch_pass1(c, d_points, d_targets, n, c->loss_scale / static_cast<float>(n), st, nullptr);
+    launch_wgrad(c, st, nullptr);
     N2_HIP(hipGetLastError());
     N2_HIP(hipDeviceSynchronize());
     const size_t gf = grid_floats(c);
@@ -782,18 +817,18 @@ extern "C" int n2wos_cache_gradients(n2wos_cache* c, const float* d_points, cons
         grid_grad[k] = s;
       }
     }
-    if (weight_grad) N2_HIP(hipMemcpy(weight_grad, c->w_grad, kCacheWeights * 4, hipMemcpyDeviceToHost));
+    if (weight_grad) N2_HIP(hipMemcpy(weight_grad, c->w_grad, static_cast<size_t>(c->n_w) * 4, hipMemcpyDeviceToHost));
     const float inv = 1.0f / c->loss_scale;
     if (grid_grad)
       for (size_t k = 0; k < gf; ++k) grid_grad[k] *= inv;
     if (weight_grad)
-      for (int k = 0; k < kCacheWeights; ++k) weight_grad[k] *= inv;
+      for (int k = 0; k < c->n_w; ++k) weight_grad[k] *= inv;
     if (loss) {
       N2_HIP(hipMemcpy(loss, c->loss, 4, hipMemcpyDeviceToHost));
       *loss /= static_cast<float>(n);
     }
     N2_HIP(hipMemset(c->grid_grad, 0, gf * 4 * static_cast<size_t>(c->grad_copies)));
-    N2_HIP(hipMemset(c->w_grad, 0, kCacheWeights * 4));
+    N2_HIP(hipMemset(c->w_grad, 0, static_cast<size_t>(c->n_w) * 4));
     return N2WOS_OK;
   });
 }

@@ -8,11 +8,14 @@
 // hash (x * 1 ^ y * 2654435761 ^ z * 805459861) mod T.  Features are
 // trilinearly interpolated from the 8 cell corners.  Tables are FP32.
 //
-// MLP: 2L = 32 inputs -> 4 hidden layers of 64 (ReLU) -> 1 output, no biases
-// (the layout of a fully fused MLP).  Weights are FP32 masters with an FP16
-// copy for the MFMA kernels (FP32 accumulation).  Layout of the weight buffer
-// (row-major, out x in): A1 [64 x 32], A2..A4 [64 x 64], A5 [16 x 64] whose
-// row 0 is the output layer and rows 1..15 are zero (MFMA tile padding).
+// MLP: 2L = 32 inputs -> `layers` hidden layers of width H (ReLU) -> 1 output,
+// no biases (the layout of a fully fused MLP); H in {16, 32, 64}, layers in
+// 1..4.  Weights are FP32 masters with an FP16 copy for the MFMA kernels
+// (FP32 accumulation).  Layout of the weight buffer (row-major, out x in):
+// A1 [H x 32] at 0, hidden layer k = 1 .. layers-1 [H x H] at
+// 32 H + (k - 1) H^2, the output block [16 x H] at 32 H + (layers - 1) H^2,
+// whose row 0 is the output layer and rows 1..15 are zero (MFMA tile
+// padding).  The default 4 x 64 network has 15,360 weights.
 #pragma once
 
 #include <cstdint>
@@ -22,15 +25,14 @@ namespace n2 {
 constexpr int kCacheLevels = 16;
 constexpr int kCacheFeat = 2;
 constexpr int kCacheIn = kCacheLevels * kCacheFeat;  // 32
-constexpr int kCacheHidden = 64;
-constexpr int kCacheW1 = kCacheHidden * kCacheIn;            // 2048
-constexpr int kCacheWh = kCacheHidden * kCacheHidden;        // 4096
-constexpr int kCacheW5 = 16 * kCacheHidden;                  // 1024 (row 0 live)
-constexpr int kCacheWeights = kCacheW1 + 3 * kCacheWh + kCacheW5;  // 15360
-constexpr int kCacheOffA2 = kCacheW1;
-constexpr int kCacheOffA3 = kCacheW1 + kCacheWh;
-constexpr int kCacheOffA4 = kCacheW1 + 2 * kCacheWh;
-constexpr int kCacheOffA5 = kCacheW1 + 3 * kCacheWh;
+constexpr int kCacheMaxHidden = 64;
+constexpr int kCacheMaxLayers = 4;
+
+constexpr int cache_weights(int h, int layers) { return kCacheIn * h + (layers - 1) * h * h + 16 * h; }
+// first weight of hidden layer k (1 .. layers-1; its input is activation k)
+constexpr int cache_off_hidden(int h, int k) { return kCacheIn * h + (k - 1) * h * h; }
+constexpr int cache_off_out(int h, int layers) { return kCacheIn * h + (layers - 1) * h * h; }
+constexpr int kCacheWeights = cache_weights(kCacheMaxHidden, kCacheMaxLayers);  // 15360, the largest network
 
 struct GridLayout {
   int n_levels;

@@ -50,12 +50,16 @@ ALIASES = {"x2_minus_y2": "harmonic_x2_minus_y2", "zebra_k4": "harmonic_zebra_k4
            "zebra_k12": "harmonic_zebra_k12", "constant": "constant_one", "one": "constant_one",
            "const1": "constant_one", "charges8": "external_charges8"}
 
-# grid part of the reference's cache presets (eval_tcnn_nc_wos.cu:249-266):
-# n_levels, log2_hashmap_size, base_resolution, per_level_scale
+# the reference's cache presets (eval_tcnn_nc_wos.cu:249-266): n_levels,
+# log2_hashmap_size, base_resolution, per_level_scale, n_neurons,
+# n_hidden_layers, n_features_per_level
 PRESETS = {
-    "baseline": (12, 18, 8, 1.5), "paper_like": (12, 18, 8, 1.5), "light": (8, 15, 8, 1.5),
-    "nano": (6, 13, 8, 1.5), "heavy": (16, 19, 8, 1.4), "xlarge": (16, 20, 8, 1.35),
+    "baseline": (12, 18, 8, 1.5, 32, 2, 2), "paper_like": (12, 18, 8, 1.5, 32, 2, 2),
+    "light": (8, 15, 8, 1.5, 16, 1, 2), "nano": (6, 13, 8, 1.5, 16, 1, 2), "heavy": (16, 19, 8, 1.4, 64, 3, 2),
+    "xlarge": (16, 20, 8, 1.35, 64, 4, 4),
 }
+PRESET_FIELDS = ("n_levels", "log2_hashmap_size", "base_resolution", "per_level_scale", "n_neurons",
+                 "n_hidden_layers", "n_features_per_level")
 
 
 def parse_args(argv=None):
@@ -101,15 +105,17 @@ def parse_args(argv=None):
     a("--epsilon", type=float, default=1e-4)
     a("--step-scale", type=float, default=0.999)
     a("--seed", type=int, default=12345)
-    a("--cache-preset", default="custom")
+    # default: the reference's "baseline" preset; an explicit encoding or
+    # network flag overrides the preset's value and makes the run "custom"
+    a("--cache-preset", default="baseline")
     a("--network", default="hashgrid_mlp_mfma")
-    a("--n-levels", type=int, default=16)
-    a("--n-features-per-level", type=int, default=2)
-    a("--log2-hashmap-size", type=int, default=19)
-    a("--base-resolution", type=int, default=16)
-    a("--per-level-scale", type=float, default=1.5)
-    a("--n-neurons", type=int, default=64)
-    a("--n-hidden-layers", type=int, default=4)
+    a("--n-levels", type=int, default=None)
+    a("--n-features-per-level", type=int, default=None)
+    a("--log2-hashmap-size", type=int, default=None)
+    a("--base-resolution", type=int, default=None)
+    a("--per-level-scale", type=float, default=None)
+    a("--n-neurons", type=int, default=None)
+    a("--n-hidden-layers", type=int, default=None)
     a("--learning-rate", type=float, default=1e-2)
     a("--batch-size", type=int, default=0, help="Adam batch (0 = the whole training set, as the reference)")
     a("--antithetic", type=int, default=0, help="mirrored first steps in NC walk pairs (this engine's option)")
@@ -121,14 +127,26 @@ def parse_args(argv=None):
     opt.bc = ALIASES.get(opt.bc, opt.bc)
     if opt.bc not in BOUNDARIES:
         ap.error(f"unsupported boundary {opt.bc!r}; this engine evaluates {sorted(BOUNDARIES)}")
-    if opt.cache_preset != "custom":
-        if opt.cache_preset not in PRESETS:
-            ap.error(f"unknown cache preset {opt.cache_preset!r} (expected {sorted(PRESETS)} or custom)")
-        opt.n_levels, opt.log2_hashmap_size, opt.base_resolution, opt.per_level_scale = PRESETS[opt.cache_preset]
+    preset = "baseline" if opt.cache_preset == "custom" else opt.cache_preset
+    if preset not in PRESETS:
+        ap.error(f"unknown cache preset {opt.cache_preset!r} (expected {sorted(PRESETS)} or custom)")
+    explicit = False
+    for name, value in zip(PRESET_FIELDS, PRESETS[preset]):
+        if getattr(opt, name) is None:
+            setattr(opt, name, value)
+        else:
+            explicit = True
+    if explicit:
+        opt.cache_preset = "custom"
     if opt.n_features_per_level != 2:
-        ap.error("this engine's hash grid has 2 features per level")
+        ap.error("this engine's hash grid has 2 features per level (preset xlarge needs 4: use "
+                 "--cache-preset heavy --n-hidden-layers 4 --log2-hashmap-size 20 --per-level-scale 1.35)")
     if not 1 <= opt.n_levels <= 16:
         ap.error("--n-levels must be in [1, 16]")
+    if opt.n_neurons not in (16, 32, 64):
+        ap.error("--n-neurons must be 16, 32 or 64")
+    if not 1 <= opt.n_hidden_layers <= 4:
+        ap.error("--n-hidden-layers must be in [1, 4]")
     return opt
 
 
@@ -347,7 +365,8 @@ def main(argv=None) -> int:
     cfg = K.CacheConfig.for_bounds(lo, hi, n_levels=opt.n_levels, log2_table_size=opt.log2_hashmap_size,
                                    base_resolution=opt.base_resolution, growth=opt.per_level_scale,
                                    learning_rate=opt.learning_rate, seed=opt.seed,
-                                   max_batch=opt.batch_size if opt.batch_size > 0 else len(train))
+                                   max_batch=opt.batch_size if opt.batch_size > 0 else len(train),
+                                   n_neurons=opt.n_neurons, n_hidden_layers=opt.n_hidden_layers)
     cache = K.Cache(cfg)
     solver = W.Solver(bvh, bc, params)
     label_ms = train_ms = 0.0
@@ -486,10 +505,12 @@ def main(argv=None) -> int:
             "coarse_walks_per_point": opt.coarse_walks_per_point,
             "residual_walks_per_point": opt.residual_walks_per_point, "depth_m": opt.depth_m,
             "max_steps": opt.max_steps, "epsilon": opt.epsilon, "cache_preset": opt.cache_preset,
-            "network": "hashgrid_relu_mlp_4x64_fp16_mfma", "n_levels": opt.n_levels,
+            "network": "hashgrid_relu_mlp_%dx%d_fp16_mfma" % (opt.n_hidden_layers, opt.n_neurons),
+            "n_levels": opt.n_levels,
             "n_features_per_level": 2, "log2_hashmap_size": opt.log2_hashmap_size,
             "base_resolution": opt.base_resolution, "per_level_scale": opt.per_level_scale,
-            "n_neurons": 64, "n_hidden_layers": 4, "jit_requested": bool(opt.jit), "jit_supported": False,
+            "n_neurons": opt.n_neurons, "n_hidden_layers": opt.n_hidden_layers, "jit_requested": bool(opt.jit),
+            "jit_supported": False,
             "jit_enabled": False,
         },
     }

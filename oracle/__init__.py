@@ -89,16 +89,16 @@ def lib():
                                      _f32p, _f32p, _i32p, _i32p, C.c_void_p, C.c_void_p]
         L.orc_cache_layout.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p]
-        L.orc_cache_forward.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _f32p, _f32p, _f32p, _f32p, _f32p,
-                                        C.c_long, _f64p]
-        L.orc_cache_grads.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _f32p, _f32p, _f32p, _f32p, _f32p,
-                                      _f32p, C.c_long, _f64p, _f64p]
+        L.orc_cache_forward.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _f32p, _f32p, _f32p,
+                                        _f32p, _f32p, C.c_long, _f64p]
+        L.orc_cache_grads.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _f32p, _f32p, _f32p,
+                                      _f32p, _f32p, _f32p, C.c_long, _f64p, _f64p]
         L.orc_cache_grads.restype = C.c_double
         L.orc_round_half.argtypes = [C.c_double]
         L.orc_round_half.restype = C.c_double
         L.orc_cache_forward_fp16.argtypes = L.orc_cache_forward.argtypes
-        L.orc_cache_encode.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _f32p, _f32p, _f32p, _f32p,
-                                       C.c_long, C.c_int, _f64p]
+        L.orc_cache_encode.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _f32p, _f32p,
+                                       _f32p, _f32p, C.c_long, C.c_int, _f64p]
         L.orc_cache_grads_fp16.argtypes = L.orc_cache_grads.argtypes + [C.c_double]
         L.orc_cache_grads_fp16.restype = C.c_double
         _lib = L
@@ -304,7 +304,7 @@ def cache_layout(n_levels, log2_t, base_res, growth):
 
 def _cfg_args(cfg):
     return (cfg.n_levels, cfg.log2_table_size, cfg.base_resolution, float(np.float32(cfg.growth)),
-            _f32(cfg.origin), _f32(cfg.extent))
+            getattr(cfg, "n_neurons", 64), getattr(cfg, "n_hidden_layers", 4), _f32(cfg.origin), _f32(cfg.extent))
 
 
 def cache_forward(cfg, grid, weights, pts, fp16=False):

@@ -1211,10 +1211,12 @@ int orc_bvh4_stack_profile(const void* nodes, const void* tris, float delta, con
 /* Learned-cache oracle in FP64: multiresolution hash-grid encoding (Mueller et
  * al. 2022: N_l = floor(N_min b^l), dense levels when (N_l + 1)^3 <= T,
  * spatial hash x*1 ^ y*2654435761 ^ z*805459861 mod T, trilinear weights) and
- * a bias-free ReLU MLP 2L -> 64 -> 64 -> 64 -> 64 -> 1, with the gradients of
- * the mean squared error by reverse accumulation.  Weight buffer layout:
- * A1 [64 x 2L] (2L = 32 columns reserved), A2..A4 [64 x 64], A5 row 0 [64]
- * (rows 1..15 of a 16 x 64 block unused), offsets 0, 2048, 6144, 10240, 14336. */
+ * a bias-free ReLU MLP 2L -> H -> ... -> H -> 1 (`layers` hidden layers of
+ * width H <= 64), with the gradients of the mean squared error by reverse
+ * accumulation.  Weight buffer layout (orc_w_off): A1 [H x 2L] (32 columns
+ * reserved), the hidden layers [H x H], the output row 0 of a 16 x H block
+ * (rows 1..15 unused); for H = 64 and 4 layers: offsets 0, 2048, 6144,
+ * 10240, 14336. */
 typedef struct {
   int n_levels;
   uint32_t table_size;
@@ -1222,7 +1224,16 @@ typedef struct {
   int res[16], dense[16];
   uint32_t total;
   double in_min[3], in_scale[3];
+  int hidden, layers; /* MLP: `layers` hidden layers of width `hidden` */
 } orc_grid;
+
+/* Weight-buffer layout of the MLP with L hidden layers of width H: A1 [H x 32]
+ * at 0, hidden layer k (k = 1 .. L-1) [H x H] at 32 H + (k - 1) H^2, the
+ * output [16 x H] (row 0 live) at 32 H + (L - 1) H^2. */
+static int orc_w_off(const orc_grid* g, int layer) {
+  return layer == 0 ? 0 : 32 * g->hidden + (layer - 1) * g->hidden * g->hidden;
+}
+static int orc_w_out(const orc_grid* g) { return 32 * g->hidden + (g->layers - 1) * g->hidden * g->hidden; }
 
 int orc_cache_layout(int n_levels, int log2_t, int base_res, double growth, uint32_t* offsets, uint32_t* sizes,
                      int* res, int* dense, uint32_t* total) {
@@ -1305,9 +1316,11 @@ static void orc_corners_f32(const orc_grid* g, int l, const float u[3], uint32_t
   }
 }
 
-static void orc_grid_init(orc_grid* g, int n_levels, int log2_t, int base_res, double growth, const float* in_min,
-                          const float* in_extent) {
+static void orc_grid_init(orc_grid* g, int n_levels, int log2_t, int base_res, double growth, int hidden, int layers,
+                          const float* in_min, const float* in_extent) {
   g->n_levels = n_levels;
+  g->hidden = hidden;
+  g->layers = layers;
   g->table_size = 1u << log2_t;
   orc_cache_layout(n_levels, log2_t, base_res, growth, g->offset, g->size, g->res, g->dense, &g->total);
   for (int a = 0; a < 3; ++a) {
@@ -1369,23 +1382,25 @@ static double orc_mlp_fwd_ex(const orc_grid* g, const float* grid, const float* 
   }
   const double* in = x;
   int nin = 32;
-  const int offs[4] = {0, 2048, 6144, 10240};
-  for (int layer = 0; layer < 4; ++layer) {
-    for (int o = 0; o < 64; ++o) {
+  const int H = g->hidden, L = g->layers;
+  for (int layer = 0; layer < L; ++layer) {
+    const int off = orc_w_off(g, layer);
+    for (int o = 0; o < H; ++o) {
       double z = 0.0;
       for (int k = 0; k < nin; ++k) {
-        const double wk = half ? orc_round_half((double)wt[offs[layer] + o * nin + k]) : (double)wt[offs[layer] + o * nin + k];
+        const double wk = half ? orc_round_half((double)wt[off + o * nin + k]) : (double)wt[off + o * nin + k];
         z += wk * in[k];
       }
       const double r = z > 0.0 ? z : 0.0;
       h[layer][o] = half ? orc_round_half(r) : r;
     }
     in = h[layer];
-    nin = 64;
+    nin = H;
   }
+  const int out = orc_w_out(g);
   double y = 0.0;
-  for (int k = 0; k < 64; ++k)
-    y += (half ? orc_round_half((double)wt[14336 + k]) : (double)wt[14336 + k]) * h[3][k];
+  for (int k = 0; k < H; ++k)
+    y += (half ? orc_round_half((double)wt[out + k]) : (double)wt[out + k]) * h[L - 1][k];
   return y;
 }
 
@@ -1397,10 +1412,10 @@ static double orc_mlp_fwd(const orc_grid* g, const float* grid, const float* wt,
 /* The hash-grid encoding alone: out[32 * s + 2 l + f] = feature f of level l
  * at point s (levels >= n_levels are 0).  f32 != 0 uses the FP32 unit-cube
  * mapping and corner weights of the product kernels, else FP64. */
-int orc_cache_encode(int n_levels, int log2_t, int base_res, double growth, const float* in_min,
+int orc_cache_encode(int n_levels, int log2_t, int base_res, double growth, int hidden, int layers, const float* in_min,
                      const float* in_extent, const float* grid, const float* pts, long n, int f32, double* out) {
   orc_grid g;
-  orc_grid_init(&g, n_levels, log2_t, base_res, growth, in_min, in_extent);
+  orc_grid_init(&g, n_levels, log2_t, base_res, growth, hidden, layers, in_min, in_extent);
   for (long s = 0; s < n; ++s) {
     double* x = out + 32 * s;
     for (int k = 0; k < 32; ++k) x[k] = 0.0;
@@ -1436,11 +1451,11 @@ int orc_cache_encode(int n_levels, int log2_t, int base_res, double growth, cons
   return 0;
 }
 
-int orc_cache_forward(int n_levels, int log2_t, int base_res, double growth, const float* in_min,
+int orc_cache_forward(int n_levels, int log2_t, int base_res, double growth, int hidden, int layers, const float* in_min,
                       const float* in_extent, const float* grid, const float* wt, const float* pts, long n,
                       double* out) {
   orc_grid g;
-  orc_grid_init(&g, n_levels, log2_t, base_res, growth, in_min, in_extent);
+  orc_grid_init(&g, n_levels, log2_t, base_res, growth, hidden, layers, in_min, in_extent);
   double x[32], h[4][64];
   for (long s = 0; s < n; ++s) out[s] = orc_mlp_fwd(&g, grid, wt, pts + 3 * s, x, h);
   return 0;
@@ -1449,7 +1464,8 @@ int orc_cache_forward(int n_levels, int log2_t, int base_res, double growth, con
 /* Gradients of L = (1/n) sum (y - t)^2 w.r.t. the grid and the weights
  * (accumulated into grid_grad[2 * total] and w_grad[15360], which the caller
  * zeroes); returns L. */
-static double orc_cache_grads_ex(int n_levels, int log2_t, int base_res, double growth, const float* in_min,
+static double orc_cache_grads_ex(int n_levels, int log2_t, int base_res, double growth, int hidden, int layers,
+                                 const float* in_min,
                                  const float* in_extent, const float* grid, const float* wt, const float* pts,
                                  const float* targets, long n, double* grid_grad, double* w_grad, double loss_scale) {
   /* loss_scale > 0 selects the FP16-rounding variant: dL/dy is scaled by
@@ -1460,8 +1476,8 @@ static double orc_cache_grads_ex(int n_levels, int log2_t, int base_res, double 
   const int half = loss_scale > 0.0;
   const double scale = half ? loss_scale : 1.0;
   orc_grid g;
-  orc_grid_init(&g, n_levels, log2_t, base_res, growth, in_min, in_extent);
-  const int offs[4] = {0, 2048, 6144, 10240};
+  orc_grid_init(&g, n_levels, log2_t, base_res, growth, hidden, layers, in_min, in_extent);
+  const int H = hidden, L = layers, out_off = orc_w_out(&g);
   double loss = 0.0;
   for (long s = 0; s < n; ++s) {
     double x[32], h[4][64];
@@ -1471,25 +1487,25 @@ static double orc_cache_grads_ex(int n_levels, int log2_t, int base_res, double 
     double dy = 2.0 * diff * (scale / (double)n);
     if (half) dy = orc_round_half(dy);
     double dh[64], dz[64];
-    for (int k = 0; k < 64; ++k) {
-      const double w5 = half ? orc_round_half((double)wt[14336 + k]) : (double)wt[14336 + k];
-      w_grad[14336 + k] += dy * h[3][k] / scale;
+    for (int k = 0; k < H; ++k) {
+      const double w5 = half ? orc_round_half((double)wt[out_off + k]) : (double)wt[out_off + k];
+      w_grad[out_off + k] += dy * h[L - 1][k] / scale;
       dh[k] = dy * w5;
     }
-    for (int layer = 3; layer >= 0; --layer) {
-      const int nin = layer == 0 ? 32 : 64;
+    for (int layer = L - 1; layer >= 0; --layer) {
+      const int nin = layer == 0 ? 32 : H;
+      const int off = orc_w_off(&g, layer);
       const double* in = layer == 0 ? x : h[layer - 1];
-      for (int o = 0; o < 64; ++o) {
+      for (int o = 0; o < H; ++o) {
         const double d = h[layer][o] > 0.0 ? dh[o] : 0.0;
         dz[o] = half ? orc_round_half(d) : d;
       }
       double din[64];
       for (int k = 0; k < nin; ++k) din[k] = 0.0;
-      for (int o = 0; o < 64; ++o) {
+      for (int o = 0; o < H; ++o) {
         for (int k = 0; k < nin; ++k) {
-          const double wk = half ? orc_round_half((double)wt[offs[layer] + o * nin + k])
-                                 : (double)wt[offs[layer] + o * nin + k];
-          w_grad[offs[layer] + o * nin + k] += dz[o] * in[k] / scale;
+          const double wk = half ? orc_round_half((double)wt[off + o * nin + k]) : (double)wt[off + o * nin + k];
+          w_grad[off + o * nin + k] += dz[o] * in[k] / scale;
           din[k] += dz[o] * wk;
         }
       }
@@ -1519,28 +1535,28 @@ static double orc_cache_grads_ex(int n_levels, int log2_t, int base_res, double 
   return loss / (double)n;
 }
 
-double orc_cache_grads(int n_levels, int log2_t, int base_res, double growth, const float* in_min,
+double orc_cache_grads(int n_levels, int log2_t, int base_res, double growth, int hidden, int layers, const float* in_min,
                        const float* in_extent, const float* grid, const float* wt, const float* pts,
                        const float* targets, long n, double* grid_grad, double* w_grad) {
-  return orc_cache_grads_ex(n_levels, log2_t, base_res, growth, in_min, in_extent, grid, wt, pts, targets, n,
+  return orc_cache_grads_ex(n_levels, log2_t, base_res, growth, hidden, layers, in_min, in_extent, grid, wt, pts, targets, n,
                             grid_grad, w_grad, 0.0);
 }
 
 /* The same gradients with the product's FP32 encoding and binary16 rounding
  * points (weights, features, activations, scaled dL/dy and dZ); loss_scale as
  * the product's. */
-double orc_cache_grads_fp16(int n_levels, int log2_t, int base_res, double growth, const float* in_min,
+double orc_cache_grads_fp16(int n_levels, int log2_t, int base_res, double growth, int hidden, int layers, const float* in_min,
                             const float* in_extent, const float* grid, const float* wt, const float* pts,
                             const float* targets, long n, double* grid_grad, double* w_grad, double loss_scale) {
-  return orc_cache_grads_ex(n_levels, log2_t, base_res, growth, in_min, in_extent, grid, wt, pts, targets, n,
+  return orc_cache_grads_ex(n_levels, log2_t, base_res, growth, hidden, layers, in_min, in_extent, grid, wt, pts, targets, n,
                             grid_grad, w_grad, loss_scale);
 }
 
-int orc_cache_forward_fp16(int n_levels, int log2_t, int base_res, double growth, const float* in_min,
+int orc_cache_forward_fp16(int n_levels, int log2_t, int base_res, double growth, int hidden, int layers, const float* in_min,
                            const float* in_extent, const float* grid, const float* wt, const float* pts, long n,
                            double* out) {
   orc_grid g;
-  orc_grid_init(&g, n_levels, log2_t, base_res, growth, in_min, in_extent);
+  orc_grid_init(&g, n_levels, log2_t, base_res, growth, hidden, layers, in_min, in_extent);
   double x[32], h[4][64];
   for (long s = 0; s < n; ++s) out[s] = orc_mlp_fwd_ex(&g, grid, wt, pts + 3 * s, x, h, 1);
   return 0;

@@ -168,3 +168,52 @@ def test_training_batch_limit_is_set_at_creation(eng, torch_gpu):
         c.train_step(torch.zeros((1009, 3), device="cuda"), torch.zeros(1009, device="cuda"))
     assert e.value.code == ERR_INVALID
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("h,layers", [(16, 1), (32, 2), (64, 3), (16, 4)])
+def test_mlp_shapes_match_fp16_oracle(eng, orc, torch_gpu, h, layers):
+    """Every MLP shape (width 16/32/64, depth 1-4): forward and MSE gradients
+    against the oracle with the kernels' binary16 rounding points (relative L2
+    <= 1e-3 per layer, 2e-3 for the grid), reproducible weight gradients, and
+    untouched padded output rows."""
+    torch = torch_gpu
+    from n2wos_amd import cache as K
+    cfg = K.CacheConfig(n_levels=8, log2_table_size=11, base_resolution=4, growth=1.5, seed=11, n_neurons=h,
+                        n_hidden_layers=layers, max_batch=2048)
+    c = K.Cache(cfg)
+    rng = np.random.default_rng(h * 10 + layers)
+    grid = rng.normal(0, 0.5, 2 * c.layout()["total_entries"]).astype(np.float32)
+    nw = K.n_weights(h, layers)
+    off = 32 * h + (layers - 1) * h * h
+    w = np.zeros(nw, np.float32)
+    w[:32 * h] = rng.normal(0, 0.25, 32 * h)
+    w[32 * h:off] = rng.normal(0, 1.4 / np.sqrt(h), off - 32 * h)
+    w[off:off + h] = rng.normal(0, 0.3, h)
+    c.set_params(grid, w)
+    g2, w2 = c.get_params()
+    assert np.array_equal(g2, grid) and np.array_equal(w2, w)
+    pts = rng.uniform(-1.05, 1.05, (1000, 3)).astype(np.float32)
+    tg = rng.normal(size=1000).astype(np.float32)
+    print("case: cache shape %dx%d" % (h, layers), flush=True)
+    y = c.infer(torch.from_numpy(pts).cuda()).cpu().numpy()
+    ref16 = orc.cache_forward(cfg, grid, w, pts, fp16=True)
+    assert np.std(ref16) > 0.05
+    assert rel(y, ref16) < 1e-3
+    gg, wg, loss = c.gradients(torch.from_numpy(pts).cuda(), torch.from_numpy(tg).cuda())
+    _, wg2, _ = c.gradients(torch.from_numpy(pts).cuda(), torch.from_numpy(tg).cuda())
+    assert np.array_equal(wg, wg2)
+    hgg, hwg, hloss = orc.cache_grads_fp16(cfg, grid, w, pts, tg, cfg.loss_scale)
+    groups = [(0, 32 * h)] + [(32 * h + k * h * h, 32 * h + (k + 1) * h * h) for k in range(layers - 1)] + \
+        [(off, off + h)]
+    print("W rel", ["%.2e" % rel(wg[a:b], hwg[a:b]) for a, b in groups], "grid %.2e" % rel(gg, hgg), flush=True)
+    assert abs(loss - hloss) < 1e-2 * hloss
+    for lo, hi in groups:
+        assert rel(wg[lo:hi], hwg[lo:hi]) < 1e-3, (lo, hi)
+    assert np.all(wg[off + h:] == 0)
+    assert rel(gg, hgg) < 2e-3
+    # one training step moves the loss down on the same batch
+    p_t, t_t = torch.from_numpy(pts).cuda(), torch.from_numpy(tg).cuda()
+    for _ in range(20):
+        c.train_step(p_t, t_t)
+    _, _, loss2 = c.gradients(p_t, t_t)
+    assert loss2 < loss

@@ -213,41 +213,69 @@ def test_welford_and_chan_against_two_pass(orc):
         assert n[p] == 250 and abs(mean[p] - seg.mean()) < 1e-12 and abs(m2[p] - ((seg - seg.mean()) ** 2).sum()) < 1e-9
 
 
-def test_cache_oracle_gradients_match_finite_differences(orc):
+def _mlp_weights(rng, h, layers):
+    """Random weights of an h-wide, layers-deep cache MLP in the product's
+    layout: A_1 [h x 32], A_2 .. A_L [h x h], A_out [16 x h] (row 0 only)."""
+    n = 32 * h + (layers - 1) * h * h + 16 * h
+    w = np.zeros(n, np.float32)
+    w[:32 * h] = rng.normal(0, 0.3, 32 * h)
+    w[32 * h:32 * h + (layers - 1) * h * h] = rng.normal(0, 1.6 / np.sqrt(h), (layers - 1) * h * h)
+    out = 32 * h + (layers - 1) * h * h
+    w[out:out + h] = rng.normal(0, 0.3, h)
+    return w
+
+
+@pytest.mark.parametrize("h,layers", [(64, 4), (16, 1), (32, 2), (64, 3)])
+def test_cache_oracle_gradients_match_finite_differences(orc, h, layers):
     """Central differences of the FP64 cache oracle's loss pin its reverse-mode
-    gradients (grid entries and weights of every layer)."""
+    gradients (grid entries and weights of every layer), for each MLP shape
+    the cache supports; the forward equals a numpy matrix chain over the
+    oracle's hash-grid features in the documented weight layout (a transposed
+    or misplaced layer fails it)."""
     from types import SimpleNamespace
     cfg = SimpleNamespace(n_levels=3, log2_table_size=8, base_resolution=2, growth=2.0, origin=(-1, -1, -1),
-                          extent=(2, 2, 2))
+                          extent=(2, 2, 2), n_neurons=h, n_hidden_layers=layers)
     lay = orc.cache_layout(3, 8, 2, 2.0)
-    rng = np.random.default_rng(0)
+    rng = np.random.default_rng(h + layers)
     grid = rng.normal(0, 0.5, 2 * lay["total_entries"]).astype(np.float32)
-    w = np.zeros(15360, np.float32)
-    w[:2048] = rng.normal(0, 0.3, 2048)
-    w[2048:14336] = rng.normal(0, 0.2, 12288)
-    w[14336:14400] = rng.normal(0, 0.3, 64)
+    w = _mlp_weights(rng, h, layers)
     pts = rng.uniform(-0.9, 0.9, (7, 3)).astype(np.float32)
     tg = rng.normal(size=7).astype(np.float32)
     gg, wg, loss = orc.cache_grads(cfg, grid, w, pts, tg)
+    assert wg.shape == w.shape
 
     def L(g, ww):
         y = orc.cache_forward(cfg, g, ww, pts)
         return np.mean((y - tg.astype(np.float64)) ** 2)
 
+    # forward: y = A_out[0] relu(A_L ... relu(A_1 x))
+    x = orc.cache_encode(cfg, grid, pts).astype(np.float64)
+    wd = w.astype(np.float64)
+    a = np.maximum(x @ wd[:32 * h].reshape(h, 32).T, 0)
+    off = 32 * h
+    for _ in range(layers - 1):
+        a = np.maximum(a @ wd[off:off + h * h].reshape(h, h).T, 0)
+        off += h * h
+    y = a @ wd[off:off + h]
+    assert np.allclose(orc.cache_forward(cfg, grid, w, pts), y, rtol=1e-12, atol=1e-12)
     assert abs(L(grid, w) - loss) < 1e-12
-    h = 1e-3
+    # steps of 1e-4 (divided by the FP32 step actually taken) keep clear of
+    # the ReLU kinks
+    eps = 1e-4
     for k in list(rng.choice(np.nonzero(gg)[0], 6, replace=False)):
         gp, gm = grid.copy(), grid.copy()
-        gp[k] += h
-        gm[k] -= h
-        fd = (L(gp, w) - L(gm, w)) / (2 * h)
+        gp[k] += eps
+        gm[k] -= eps
+        fd = (L(gp, w) - L(gm, w)) / (float(gp[k]) - float(gm[k]))
         assert abs(fd - gg[k]) < 1e-4 * max(1.0, abs(fd))
-    for k in [5, 100, 2048 + 7, 6144 + 300, 10240 + 4000, 14336 + 3]:
+    picks = [5, 32 * h - 3, off + 3] + [32 * h + j * h * h + 7 * j + 11 for j in range(layers - 1)]
+    for k in picks:
         wp, wm = w.copy(), w.copy()
-        wp[k] += h
-        wm[k] -= h
-        fd = (L(grid, wp) - L(grid, wm)) / (2 * h)
+        wp[k] += eps
+        wm[k] -= eps
+        fd = (L(grid, wp) - L(grid, wm)) / (float(wp[k]) - float(wm[k]))
         assert abs(fd - wg[k]) < 1e-4 * max(1.0, abs(fd))
+    assert np.all(wg[off + h:] == 0)  # padded rows of the output layer
 
 
 def test_cache_oracle_layout_closed_form(orc):
