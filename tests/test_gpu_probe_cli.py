@@ -67,3 +67,33 @@ def test_geometry_backend_sweep(eng, torch_gpu, tmp_path, mode):
         assert val["max_backend_distance_delta"] == 0.0, name
     best = min(b, key=lambda k: b[k]["timing"]["median_us_per_query"])
     assert j["recommendation"]["selected_production_candidate"] == best
+
+
+def test_cache_probe_reference_defaults(eng, torch_gpu, tmp_path):
+    """The learned-cache probe at the reference probe's default workload
+    (262,144 samples and batch, 200 Adam steps, 12 levels, 2^18 table, base 8,
+    scale 1.5, 32 x 2 MLP, x^2 - y^2 on the unit cube): exits 0, writes the
+    n2wos_tcnn_cache_probe_v1 keys, and the trained cache fits the target
+    (RMSE < 0.05 over 20 repeats; mean of x^2 - y^2 over the cube is 0)."""
+    exe = os.path.join(os.path.dirname(eng.LIB_PATH), "..", "bin", "n2wos_probe_tcnn_cache")
+    out = str(tmp_path / "cache_probe.json")
+    print("case: cache probe defaults", flush=True)
+    r = subprocess.run([exe, "--output", out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    j = json.load(open(out))
+    assert j["schema"] == "n2wos_tcnn_cache_probe_v1"
+    for k in ("patch", "generated_at_utc", "command_line", "cuda", "implementation_mode", "options", "training",
+              "inference", "inference_plus_downstream_consumer", "validation"):
+        assert k in j, k
+    o = j["options"]
+    assert (o["samples"], o["batch_size"], o["train_steps"], o["repeat"]) == (262144, 262144, 200, 20)
+    assert (o["n_levels"], o["log2_hashmap_size"], o["n_neurons"], o["n_hidden_layers"]) == (12, 18, 32, 2)
+    assert j["implementation_mode"]["cache_backend"] == "hip_hashgrid_mlp_mfma_gfx950"
+    assert len(j["inference"]["kernel_ms"]) == 20 and j["inference"]["median_msamples_per_second"] > 0
+    v = j["validation"]
+    assert v["checked_repeated_samples"] == 262144 * 20
+    assert abs(v["mean_exact"]) < 0.01
+    assert abs(v["mean_prediction"] - v["mean_exact"]) < 0.01
+    assert v["rmse"] < 0.05 and v["mae"] <= v["rmse"]
+    print("cache probe: rmse %.4f, train %.3f ms/step, inference %.3f us/sample" %
+          (v["rmse"], j["training"]["ms_per_step"], j["inference"]["median_us_per_sample"]), flush=True)
