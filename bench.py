@@ -366,9 +366,10 @@ def main():
     events = [hiptime.EventPair() for _ in range(args.steps)]
 
     if args.section == "walker":
-        solver.solve(tp, args.wpp, seed=params.seed)
+        st = solver.solve(tp, args.wpp, seed=params.seed)
         torch.cuda.synchronize()
-        print(json.dumps({"section": "walker", "points": n, "walks_per_point": args.wpp}))
+        print(json.dumps({"section": "walker", "points": n, "walks_per_point": args.wpp,
+                          "walk_steps": int(st.steps.sum().item()), "source_sha256": walker_source_hash()}))
         return
     if args.section == "cache":
         print(json.dumps({"section": "cache", **cache_rates(eng, torch)}))

@@ -19,6 +19,8 @@
  *                                    CuBqlBvh::CuBqlBvh (include/n2wos/cubql_bvh.hpp:33-35)
  *   n2wos_bvh_query_device           CudaBvh::query_device (include/n2wos/cuda_bvh.hpp:50-57,
  *                                    src/cuda/bvh_query.cu:228-261)
+ *   n2wos_bvh_query_host             CudaBvh::query (include/n2wos/cuda_bvh.hpp:44,
+ *                                    src/cuda/bvh_query.cu:263-335)
  *   n2wos_bvh_query_device_masked    CuBqlBvh::query_device_masked
  *                                    (include/n2wos/cubql_bvh.hpp:57-65,
  *                                    src/cuda/cubql_bvh.cu:680-714)
@@ -144,6 +146,15 @@ int n2wos_bvh_get_stats(const n2wos_bvh* bvh, n2wos_bvh_stats* out);
 int n2wos_bvh_query_device(const n2wos_bvh* bvh, const float* d_points, int query_count, float* d_distance2,
                            float* d_closest, int* d_triangle_id, int* d_overflow, int block_size,
                            void* stream);
+/* Host-memory convenience form of n2wos_bvh_query_device (synchronous):
+ * copies the points to the device, runs the query on the null stream and
+ * copies the results back; *overflow_count receives the number of overflowed
+ * queries and *kernel_milliseconds (optional) the query launch's duration
+ * measured with HIP events.  Allocates and frees its device buffers: for
+ * probes and tests, not for a loop (use n2wos_bvh_query_device there). */
+int n2wos_bvh_query_host(const n2wos_bvh* bvh, const float* points, int query_count, float* distance2,
+                         float* closest, int* triangle_id, int* overflow_count, float* kernel_milliseconds,
+                         int block_size);
 /* As above; where d_active[i] == 0 the outputs are d2 = 0, closest = the
  * point itself, tri = -1, overflow = 0 and no traversal runs. */
 int n2wos_bvh_query_device_masked(const n2wos_bvh* bvh, const float* d_points, const uint8_t* d_active,

@@ -71,6 +71,7 @@ _SIGNATURES = {
     "n2wos_bvh_destroy": ([vp], i32),
     "n2wos_bvh_get_stats": ([vp, C.POINTER(BvhStats)], i32),
     "n2wos_bvh_query_device": ([vp, vp, i32, vp, vp, vp, vp, i32, vp], i32),
+    "n2wos_bvh_query_host": ([vp, vp, i32, vp, vp, vp, C.POINTER(i32), C.POINTER(f32), i32], i32),
     "n2wos_bvh_query_device_masked": ([vp, vp, vp, i32, vp, vp, vp, vp, i32, vp], i32),
     "n2wos_bvh_query_device_counted": ([vp, vp, i32, vp, vp, vp, vp, vp, vp, i32, vp], i32),
     "n2wos_bvh_build_host": ([vp, i32, vp, i32, i32, C.c_char_p, vp, C.POINTER(i64), vp, C.POINTER(f32)], i32),

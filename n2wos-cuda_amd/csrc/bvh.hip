@@ -1,9 +1,10 @@
 // bvh.hip -- BVH handle and the closest-point query kernels behind
-// n2wos_bvh_query_device[_masked|_counted].
+// n2wos_bvh_query_device[_masked|_counted] and n2wos_bvh_query_host.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
 #include <cstring>
+#include <vector>
 
 #include "api_internal.hpp"
 #include "traverse.hpp"
@@ -326,6 +327,61 @@ extern "C" int n2wos_bvh_query_device(const n2wos_bvh* bvh, const float* d_point
   return guarded([&]() -> int {
     return launch_query<false, false>(bvh, d_points, nullptr, query_count, d_distance2, d_closest, d_triangle_id,
                                       d_overflow, nullptr, nullptr, block_size, stream);
+  });
+}
+
+extern "C" int n2wos_bvh_query_host(const n2wos_bvh* bvh, const float* points, int query_count, float* distance2,
+                                    float* closest, int* triangle_id, int* overflow_count,
+                                    float* kernel_milliseconds, int block_size) {
+  return guarded([&]() -> int {
+    N2_REQUIRE(bvh != nullptr, N2WOS_ERR_NULL, "bvh handle is NULL");
+    N2_REQUIRE(query_count >= 0, N2WOS_ERR_INVALID, "query_count must be non-negative");
+    if (overflow_count) *overflow_count = 0;
+    if (kernel_milliseconds) *kernel_milliseconds = 0.0f;
+    if (query_count == 0) return N2WOS_OK;
+    N2_REQUIRE(points && distance2 && closest && triangle_id, N2WOS_ERR_NULL, "NULL host pointer");
+    const size_t n = static_cast<size_t>(query_count);
+    // one allocation: points, closest (3 floats each), d2, tri, overflow
+    void* buf = nullptr;
+    N2_HIP(hipMalloc(&buf, n * 36));
+    struct Free {
+      void* p;
+      ~Free() { (void)hipFree(p); }
+    } guard{buf};
+    float* d_pts = static_cast<float*>(buf);
+    float* d_cp = d_pts + 3 * n;
+    float* d_d2 = d_cp + 3 * n;
+    int* d_tri = reinterpret_cast<int*>(d_d2 + n);
+    int* d_ovf = d_tri + n;
+    N2_HIP(hipMemcpy(d_pts, points, n * 12, hipMemcpyHostToDevice));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    N2_HIP(hipEventCreate(&e0));
+    N2_HIP(hipEventCreate(&e1));
+    struct Ev {
+      hipEvent_t a, b;
+      ~Ev() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+      }
+    } evs{e0, e1};
+    N2_HIP(hipEventRecord(e0, nullptr));
+    const int rc = launch_query<false, false>(bvh, d_pts, nullptr, query_count, d_d2, d_cp, d_tri, d_ovf, nullptr,
+                                              nullptr, block_size, nullptr);
+    if (rc != N2WOS_OK) return rc;
+    N2_HIP(hipEventRecord(e1, nullptr));
+    N2_HIP(hipEventSynchronize(e1));
+    if (kernel_milliseconds) N2_HIP(hipEventElapsedTime(kernel_milliseconds, e0, e1));
+    N2_HIP(hipMemcpy(distance2, d_d2, n * 4, hipMemcpyDeviceToHost));
+    N2_HIP(hipMemcpy(closest, d_cp, n * 12, hipMemcpyDeviceToHost));
+    N2_HIP(hipMemcpy(triangle_id, d_tri, n * 4, hipMemcpyDeviceToHost));
+    if (overflow_count) {
+      std::vector<int> ovf(n);
+      N2_HIP(hipMemcpy(ovf.data(), d_ovf, n * 4, hipMemcpyDeviceToHost));
+      int c = 0;
+      for (int v : ovf) c += v != 0;
+      *overflow_count = c;
+    }
+    return N2WOS_OK;
   });
 }
 

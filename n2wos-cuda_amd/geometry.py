@@ -174,6 +174,19 @@ class Bvh:
                                                       ovf.data_ptr(), block_size, s))
         return d2, cp, tri, ovf
 
+    def query_host(self, points_np, block_size=128):
+        """The C ABI's host-memory query (n2wos_bvh_query_host; CudaBvh::query,
+        bvh_query.cu:263-335): (d2, closest, tri, overflow_count, kernel_ms)."""
+        pts = np.ascontiguousarray(points_np, np.float32).reshape(-1, 3)
+        n = len(pts)
+        d2 = np.zeros(n, np.float32)
+        cp = np.zeros((n, 3), np.float32)
+        tri = np.zeros(n, np.int32)
+        ovf, ms = C.c_int(0), C.c_float(0.0)
+        check(lib().n2wos_bvh_query_host(self.handle, pts.ctypes.data, n, d2.ctypes.data, cp.ctypes.data,
+                                         tri.ctypes.data, C.byref(ovf), C.byref(ms), block_size))
+        return d2, cp, tri, ovf.value, ms.value
+
     def query(self, points_np, block_size=128):
         """Host-vector convenience wrapper (CudaBvh::query, bvh_query.cu:263-335)."""
         import torch

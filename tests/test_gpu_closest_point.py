@@ -161,3 +161,24 @@ def test_quad_and_per_lane_kernels_agree(eng, torch_gpu, c1):
         assert np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32))
     d2, cp, tri, ovf, nn, nt = res[1][:6]
     assert tri[17] == -1 and tri[1030] == -1 and ovf.sum() == 0 and nn.sum() > 0
+
+
+def test_host_vector_query_equals_device_query(eng, torch_gpu, c1):
+    """n2wos_bvh_query_host (the host-memory form, CudaBvh::query) returns the
+    device query's results bit for bit, counts overflows, times the launch,
+    and handles an empty set and a bad block size."""
+    vn, t, bvh = c1
+    rng = np.random.default_rng(9)
+    pts = rng.uniform(-1.3, 1.3, (20000, 3)).astype(np.float32)
+    pts[7] = np.nan
+    print("case: host-vector query 20000", flush=True)
+    d2, cp, tri, novf, ms = bvh.query_host(pts)
+    d2d, cpd, trid, ovfd = bvh.query(pts)
+    assert np.array_equal(bits(d2), bits(d2d)) and np.array_equal(bits(cp), bits(cpd))
+    assert np.array_equal(tri, trid) and novf == int((ovfd != 0).sum()) == 0
+    assert tri[7] == -1 and ms > 0
+    e = bvh.query_host(np.zeros((0, 3), np.float32))
+    assert len(e[0]) == 0 and e[3] == 0
+    with pytest.raises(eng.N2wosError) as err:
+        bvh.query_host(pts[:4], block_size=0)
+    assert err.value.code == eng._lib.ERR_BLOCK
