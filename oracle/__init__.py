@@ -79,6 +79,11 @@ def lib():
         L.orc_walks.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_int, _f32p, C.c_int, C.c_uint64,
                                 C.c_uint32, _f32p, C.c_void_p, C.c_long, C.c_int, C.c_int, _f32p, _i32p,
                                 C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+        L.orc_cv_term.argtypes = [_f32p, C.c_int, _f32p, _f32p, C.c_int, _f32p, C.c_float, _f32p, C.c_uint32]
+        L.orc_cv_term.restype = C.c_float
+        L.orc_cv_lookup.argtypes = [_f32p, C.c_int, _f32p, _f32p, _f32p]
+        L.orc_cv_lookup.restype = C.c_float
+        L.orc_cv_frame.argtypes = [_f32p, C.c_float, _f32p, _f32p]
         L.orc_walks_cv.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_int, _f32p, C.c_int, C.c_uint64,
                                    C.c_uint32, _f32p, C.c_long, C.c_int, _f32p, _i32p, C.c_int, C.c_int, C.c_int,
                                    _f32p, _f32p, _f32p, C.c_float]
@@ -248,6 +253,26 @@ class Bvh2:
         if rc != 0:
             raise ValueError("orc_walks_cv failed")
         return values, steps
+
+
+def cv_term(field, points, p, R, d, psi_word=0):
+    """One control-variate term C(p + R d) - (1/m) sum_j C(p + R s_j) of the
+    walker (m = points: 2 antipodal, 6 turned octahedron); field as walks_cv."""
+    grid, res, origin, inv_h = field
+    return lib().orc_cv_term(_f32(grid).ravel(), res, _f32(origin), _f32(inv_h), points, _f32(p), R, _f32(d),
+                             psi_word)
+
+
+def cv_lookup(field, q):
+    grid, res, origin, inv_h = field
+    return lib().orc_cv_lookup(_f32(grid).ravel(), res, _f32(origin), _f32(inv_h), _f32(q))
+
+
+def cv_frame(d, psi):
+    e1 = np.zeros(3, np.float32)
+    e2 = np.zeros(3, np.float32)
+    lib().orc_cv_frame(_f32(d), psi, e1, e2)
+    return e1, e2
 
 
 def point_stats(values, npts: int, wpp: int):

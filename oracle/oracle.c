@@ -36,6 +36,24 @@
  *   orc_philox4x32_10              Salmon et al., SC'11 (Random123)
  *   orc_sincos2pi                  polynomial sin/cos of 2*pi*v used by both
  *                                  the product and this oracle
+ *   orc_cache_*                    the learned cache (hash grid + ReLU MLP) that
+ *                                  replaces tiny-cuda-nn's HashGrid +
+ *                                  FullyFusedMLP (src/tools/eval_tcnn_nc_wos.cu:
+ *                                  747-769); FP64, plus a variant with the
+ *                                  kernels' binary16 rounding points
+ *   orc_walks_cv, orc_cv_*         walks with the along-walk control variate (no
+ *                                  reference counterpart: a variance reduction
+ *                                  of this engine)
+ *
+ * How each part is pinned (tests/test_oracle.py and the other CPU tests): the
+ * compiled reference core and its golden vectors (closest point, brute force,
+ * meshes, normalisation), known-answer vectors (Philox), closed forms
+ * (triangle regions, radius formulas at the vertices, trilinear reproduction
+ * of polynomials by the hash grid and by the control-variate field, the
+ * control-variate term of quadratic fields, frame orthonormality), invariants
+ * and statistics (zero-mean control variate, calibrated walk estimates against
+ * FP64 solutions), finite differences (cache gradients) and two-pass FP64
+ * (Welford / Chan).
  */
 #include <float.h>
 #include <math.h>
@@ -805,6 +823,39 @@ static float cv_term(const orc_walk_opts* o, v3 p, float R, v3 d, uint32_t psi_w
     sum = sum + cv_lookup(o, p.x - R * e2.x, p.y - R * e2.y, p.z - R * e2.z);
   }
   return a - sum / (float)o->cv_points;
+}
+
+/* Test hooks: one control-variate term and one frame, for the closed-form
+ * checks of the CPU tests. */
+float orc_cv_term(const float* grid, int res, const float* origin, const float* inv_h, int points, const float* p,
+                  float R, const float* d, uint32_t psi_word) {
+  orc_walk_opts o = {0, 0.0f, 0.0f, 0, NULL, 0, 0u, 0u, 0u, 0, 0, 0, NULL, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, 0.0f};
+  o.cv_points = points;
+  o.cv_res = res;
+  o.cv_grid = grid;
+  for (int a = 0; a < 3; ++a) {
+    o.cv_origin[a] = origin[a];
+    o.cv_inv_h[a] = inv_h[a];
+  }
+  return cv_term(&o, mk(p[0], p[1], p[2]), R, mk(d[0], d[1], d[2]), psi_word);
+}
+
+float orc_cv_lookup(const float* grid, int res, const float* origin, const float* inv_h, const float* q) {
+  orc_walk_opts o = {0, 0.0f, 0.0f, 0, NULL, 0, 0u, 0u, 0u, 0, 0, 0, NULL, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, 0.0f};
+  o.cv_res = res;
+  o.cv_grid = grid;
+  for (int a = 0; a < 3; ++a) {
+    o.cv_origin[a] = origin[a];
+    o.cv_inv_h[a] = inv_h[a];
+  }
+  return cv_lookup(&o, q[0], q[1], q[2]);
+}
+
+void orc_cv_frame(const float* d, float psi, float* e1, float* e2) {
+  v3 a, b;
+  cv_frame(mk(d[0], d[1], d[2]), psi, &a, &b);
+  e1[0] = a.x; e1[1] = a.y; e1[2] = a.z;
+  e2[0] = b.x; e2[1] = b.y; e2[2] = b.z;
 }
 
 static inline float median3(float a, float b, float c) {

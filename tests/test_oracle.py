@@ -405,3 +405,93 @@ def test_hash_grid_vertex_values_and_spatial_hash(orc, eng, f32):
             e = lay["offsets"][lvl] + local
             for f in range(2):
                 assert abs(enc[k, 2 * lvl + f] - grid[2 * e + f]) < (2e-6 if not f32 else 2e-5), (lvl, x, y, z, f)
+
+
+def _poly_field(res, coef, lo=-1.5, hi=1.5):
+    """A trilinear polynomial c0 + a.x + k1 xy + k2 yz + k3 xz + k4 xyz sampled
+    at the vertices of a res^3 grid over [lo, hi]^3 (x fastest), as the
+    walker's control-variate field (values, res, origin, inverse cell size)."""
+    c0, a, k1, k2, k3, k4 = coef
+    ax = lo + (hi - lo) * np.arange(res + 1) / res
+    Z, Y, X = np.meshgrid(ax, ax, ax, indexing="ij")
+    vals = c0 + a[0] * X + a[1] * Y + a[2] * Z + k1 * X * Y + k2 * Y * Z + k3 * X * Z + k4 * X * Y * Z
+    origin = np.full(3, lo, np.float32)
+    inv = np.full(3, res / (hi - lo), np.float32)
+    return (np.ascontiguousarray(vals.ravel(), np.float32), res, origin, inv)
+
+
+def test_cv_lookup_reproduces_trilinear_polynomials(orc):
+    """Trilinear interpolation reproduces any trilinear polynomial exactly, so
+    the oracle's field lookup must return it at arbitrary points (up to FP32
+    rounding): a wrong corner, weight or axis stride fails."""
+    rng = np.random.default_rng(31)
+    coef = (0.3, rng.normal(size=3), 0.7, -0.4, 0.25, 0.5)
+    field = _poly_field(24, coef)
+    for q in rng.uniform(-1.45, 1.45, (200, 3)).astype(np.float32):
+        x, y, z = q.astype(np.float64)
+        want = coef[0] + coef[1] @ q.astype(np.float64) + coef[2] * x * y + coef[3] * y * z + coef[4] * x * z + \
+            coef[5] * x * y * z
+        assert abs(orc.cv_lookup(field, q) - want) < 2e-5, q
+
+
+def test_cv_frame_orthonormal_and_turned(orc):
+    """The frame (d, e1, e2) is orthonormal for every direction (both
+    hemispheres, the poles), and psi turns e1 about d by 2 pi psi."""
+    rng = np.random.default_rng(32)
+    ds = rng.normal(size=(300, 3))
+    ds = np.vstack([ds, [[0, 0, 1], [0, 0, -1], [1, 0, 0], [0, -1, 0]]])
+    ds = (ds / np.linalg.norm(ds, axis=1, keepdims=True)).astype(np.float32)
+    for d in ds:
+        psi = float(rng.uniform())
+        e1, e2 = orc.cv_frame(d, psi)
+        M = np.stack([d, e1, e2]).astype(np.float64)
+        assert np.allclose(M @ M.T, np.eye(3), atol=3e-6), d
+        f1, _ = orc.cv_frame(d, 0.0)
+        assert abs(float(f1.astype(np.float64) @ e1) - np.cos(2 * np.pi * psi)) < 3e-6
+
+
+@pytest.mark.parametrize("points", [2, 6])
+def test_cv_term_closed_forms(orc, points):
+    """For a quadratic field C(x) = c0 + g.x + x^T A x with zero-diagonal A
+    (bilinear terms, reproduced exactly by the trilinear grid), the term of
+    one step is C(p + R d) minus the mean of C over the design:
+      antipodal (m = 2):   R grad C(p) . d
+      octahedron (m = 6):  R grad C(p) . d + R^2 d^T A d  (the frame's three
+                           axes sum d^T A d + e1^T A e1 + e2^T A e2 = tr A = 0).
+    A dropped design point, a wrong sign or a mis-scaled mean fails this."""
+    rng = np.random.default_rng(33 + points)
+    g = rng.normal(size=3)
+    k1, k2, k3 = 0.6, -0.8, 0.45
+    field = _poly_field(32, (0.2, g, k1, k2, k3, 0.0))
+    A = np.array([[0, k1 / 2, k3 / 2], [k1 / 2, 0, k2 / 2], [k3 / 2, k2 / 2, 0]])
+    for _ in range(200):
+        p = rng.uniform(-0.9, 0.9, 3).astype(np.float32)
+        d = rng.normal(size=3)
+        d = (d / np.linalg.norm(d)).astype(np.float32)
+        R = np.float32(rng.uniform(0.05, 0.5))
+        psi_word = int(rng.integers(0, 2 ** 32))
+        t = orc.cv_term(field, points, p, R, d, psi_word)
+        pd, dd, Rd = p.astype(np.float64), d.astype(np.float64), float(R)
+        want = Rd * (g + 2 * A @ pd) @ dd
+        if points == 6:
+            want += Rd * Rd * dd @ A @ dd
+        assert abs(t - want) < 3e-5, (p, d, R, t, want)
+
+
+def test_cv_term_zero_mean_for_any_field(orc):
+    """Every design point is uniform on the sphere of radius R when d and psi
+    are, so the term has mean zero for ANY field: here a rough random grid,
+    at a fixed point, over 20,000 uniform (d, psi) -- the mean is within 4
+    standard errors of 0 for both designs."""
+    rng = np.random.default_rng(35)
+    res = 16
+    vals = rng.normal(size=(res + 1) ** 3).astype(np.float32)
+    field = (vals, res, np.full(3, -1.0, np.float32), np.full(3, res / 2.0, np.float32))
+    p = np.array([0.13, -0.21, 0.07], np.float32)
+    for points in (2, 6):
+        d = rng.normal(size=(20000, 3))
+        d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+        words = rng.integers(0, 2 ** 32, 20000)
+        t = np.array([orc.cv_term(field, points, p, 0.6, d[k], int(words[k])) for k in range(20000)])
+        assert np.std(t) > 0.05
+        assert abs(t.mean()) < 4 * t.std() / np.sqrt(len(t)), points
