@@ -109,6 +109,12 @@ def run_cv(bvh, verts, points, boundary, truth, params, plain_time, tau=1e-2, se
     tp = torch.from_numpy(points).cuda()
     cs = CV.CvSolver(bvh, boundary, params)
     cs.solve(tp[:1024].contiguous(), 1, None, path_len=4)  # warm the extended walker
+    # warm every kernel of the pipeline (first launches load code objects):
+    # a tiny run on 2,048 points, not reused
+    su0 = setups[0]
+    CV.train_field(bvh, verts, boundary, params, tp[:2048].contiguous(), su0.cache_kw, 1, su0.path_len, 1, (2, 2),
+                   1024, 8, su0.points, seed + 7, solver=cs)
+    torch.cuda.synchronize()
     out = []
     best = None
     for su in setups:
