@@ -61,6 +61,28 @@ def test_eval_tool_slice_outputs(tmp_path, torch_gpu):
         assert len(f.read().strip().splitlines()) == 4096 + 1
 
 
+def test_eval_tool_nano_preset_network(tmp_path, torch_gpu):
+    """--cache-preset nano runs the reference preset's own network (6 levels,
+    2^13 table, base 8, scale 1.5, 16 x 1 MLP) and the JSON says so; NC and
+    2LMC estimates stay near the FP64 solution."""
+    out = tmp_path / "nano.json"
+    args = [sys.executable, TOOL, "--mesh", "mesh1", "--bc", "harmonic_x2_minus_y2", "--eval-mode", "ball",
+            "--eval-points", "2048", "--train-points", "4096", "--label-refreshes", "2",
+            "--walks-per-label-refresh", "16", "--train-steps-per-refresh", "100", "--cache-preset", "nano",
+            "--pure-walks-per-point", "64", "--hybrid-walks-per-point", "32", "--coarse-walks-per-point", "32",
+            "--residual-walks-per-point", "16", "--depth-m", "1", "--epsilon", "1e-3", "--output", str(out)]
+    print("case: eval tool nano preset", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    o = json.loads(out.read_text())["options"]
+    assert o["cache_preset"] == "nano"
+    assert (o["n_levels"], o["log2_hashmap_size"], o["base_resolution"], o["n_neurons"], o["n_hidden_layers"]) == \
+        (6, 13, 8, 16, 1)
+    assert o["network"] == "hashgrid_relu_mlp_1x16_fp16_mfma"
+    runs = json.loads(out.read_text())["runs"]
+    assert runs["pure_wos"]["rmse"] < 0.05 and runs["nc_2lmc"]["rmse"] < 0.1
+
+
 def test_slice_mask_equals_reference_run(eng, torch_gpu):
     """The slice sampler on the reference's own evaluation setting (bumpy
     sphere 128 x 256, amplitude 0.15, normalised, xy slice at z = 0, 512 x 512,

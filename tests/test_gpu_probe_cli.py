@@ -37,7 +37,8 @@ def test_probe_cli_problem1(eng, torch_gpu, tmp_path):
     assert len(j["timing"]["kernel_ms"]) == 3 and j["timing"]["median_mqueries_per_second"] > 0
 
 
-@pytest.mark.parametrize("mode", ["wos_like_prefix", "near_boundary_shell"])
+@pytest.mark.parametrize("mode", ["wos_like_prefix", "near_boundary_shell", "uniform_box", "interior_slice",
+                                  "surface_exact"])
 def test_geometry_backend_sweep(eng, torch_gpu, tmp_path, mode):
     """The builder / leaf-size sweep probe: every backend (median and SAH
     builders x two leaf sizes, both query kernels) validates bit for bit
@@ -97,3 +98,25 @@ def test_cache_probe_reference_defaults(eng, torch_gpu, tmp_path):
     assert v["rmse"] < 0.05 and v["mae"] <= v["rmse"]
     print("cache probe: rmse %.4f, train %.3f ms/step, inference %.3f us/sample" %
           (v["rmse"], j["training"]["ms_per_step"], j["inference"]["median_us_per_sample"]), flush=True)
+
+
+@pytest.mark.parametrize("shape", [("6", "13", "16", "1"), ("16", "19", "64", "3")])
+def test_cache_probe_preset_shapes(eng, torch_gpu, tmp_path, shape):
+    """The probe at the reference's nano (6 levels, 2^13, 16 x 1) and heavy
+    (16 levels, 2^19, 64 x 3) shapes on a smaller workload: the JSON reports
+    the shape that ran and the cache still fits x^2 - y^2."""
+    exe = os.path.join(os.path.dirname(eng.LIB_PATH), "..", "bin", "n2wos_probe_tcnn_cache")
+    out = str(tmp_path / "cache_probe.json")
+    lv, t2, h, nl = shape
+    args = [exe, "--samples", "65536", "--batch-size", "65536", "--train-steps", "150", "--repeat", "3",
+            "--n-levels", lv, "--log2-hashmap-size", t2, "--n-neurons", h, "--n-hidden-layers", nl, "--output", out]
+    print("case: cache probe shape %s" % (shape,), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    j = json.load(open(out))
+    o = j["options"]
+    assert (o["n_levels"], o["log2_hashmap_size"], o["n_neurons"], o["n_hidden_layers"]) == tuple(int(x) for x in shape)
+    assert o["network_backend"] == "hashgrid_relu_mlp_%sx%s_fp16_mfma" % (nl, h)
+    assert j["validation"]["rmse"] < 0.05
+    bad = subprocess.run([exe, "--n-neurons", "48", "--output", out], capture_output=True, text=True, timeout=60)
+    assert bad.returncode == 1 and "n_neurons" in bad.stderr
