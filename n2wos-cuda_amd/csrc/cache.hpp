@@ -32,7 +32,6 @@ constexpr int cache_weights(int h, int layers) { return kCacheIn * h + (layers -
 // first weight of hidden layer k (1 .. layers-1; its input is activation k)
 constexpr int cache_off_hidden(int h, int k) { return kCacheIn * h + (k - 1) * h * h; }
 constexpr int cache_off_out(int h, int layers) { return kCacheIn * h + (layers - 1) * h * h; }
-constexpr int kCacheWeights = cache_weights(kCacheMaxHidden, kCacheMaxLayers);  // 15360, the largest network
 
 struct GridLayout {
   int n_levels;

@@ -14,9 +14,10 @@ schedule: label_refreshes rounds, each adding walks_per_label_refresh WoS
 walks per training point to a running label (the per-point mean over all
 rounds, nc_update_labels_kernel, src/cuda/cubql_bvh.cu:1372-1420) or using
 the analytic solution, then train_steps_per_refresh Adam steps on the whole
-training set.  The cache is this engine's hash grid + 4 x 64 ReLU MLP on
-MFMA; its architecture flags other than the grid's are reported, not used.
-Exact values are the FP64 analytic solutions.
+training set.  The cache is this engine's hash grid + ReLU MLP on MFMA with
+the preset's (or the flags') grid and network shape; `--network` and
+`--jit` are reported, not used.  Exact values are the FP64 analytic
+solutions.
 
 Run as: python n2wos-cuda_amd/tools/n2wos_eval_nc_wos.py [flags]
 """

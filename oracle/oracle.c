@@ -1513,8 +1513,8 @@ int orc_cache_forward(int n_levels, int log2_t, int base_res, double growth, int
 }
 
 /* Gradients of L = (1/n) sum (y - t)^2 w.r.t. the grid and the weights
- * (accumulated into grid_grad[2 * total] and w_grad[15360], which the caller
- * zeroes); returns L. */
+ * (accumulated into grid_grad[2 * total] and w_grad[32 H + (L - 1) H^2 + 16 H],
+ * which the caller zeroes); returns L. */
 static double orc_cache_grads_ex(int n_levels, int log2_t, int base_res, double growth, int hidden, int layers,
                                  const float* in_min,
                                  const float* in_extent, const float* grid, const float* wt, const float* pts,
