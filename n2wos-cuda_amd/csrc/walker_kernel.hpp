@@ -304,7 +304,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
             break;
           }
           int pt, chunk, ucnt;
-          if (EXT && KA->pack_shift >= 0) {
+          if (KA->pack_shift >= 0) {
             // packed unit: 64 / wpp whole points
             pt = u << (6 - KA->pack_shift);
             chunk = 0;
@@ -335,7 +335,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
           // every walk of a point shares its first query: computed once per point
           slot = fill_slot;
           wiu = fill_next + r;
-          if (EXT && KA->pack_shift >= 0) {
+          if (KA->pack_shift >= 0) {
             point = fill_point + (wiu >> KA->pack_shift);
             walk = wiu & ((1 << KA->pack_shift) - 1);
           } else {
@@ -436,7 +436,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
               }
             }
             s_val[wv][slot][wiu] = value;
-            if (EXT && KA->pack_shift >= 0) {
+            if (KA->pack_shift >= 0) {
               // packed unit: the point's own counters (integer sums, any order)
               atomicAdd(&KA->unit_steps[point], static_cast<uint32_t>(step));
               if (forced || ovf)
@@ -533,7 +533,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
         done &= done - 1;
         const int s = __shfl(slot, l);
         const int u = s_unit[wv][s];
-        const bool packed = EXT && KA->pack_shift >= 0;
+        const bool packed = KA->pack_shift >= 0;
         int pt, ucnt, span;
         if (packed) {
           pt = u << (6 - KA->pack_shift);

@@ -86,8 +86,7 @@ __global__ void k_zero_u32(uint32_t* p, int n) {
   if (i < n) p[i] = 0u;
 }
 
-// Packed units of the extended walker: walks_per_point a power of two <= 32
-// (log2 of it, else -1).
+// Packed units: walks_per_point a power of two <= 32 (log2 of it, else -1).
 int pack_shift_for(int walks_per_point) {
   for (int k = 0; k <= 5; ++k)
     if (walks_per_point == (1 << k)) return k;
@@ -183,7 +182,10 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
   a.units_per_point = (r.walks_per_point + 63) / 64;
   a.total_units = r.n_points * a.units_per_point;
   a.n_points = r.n_points;
-  a.pack_shift = r.ext ? pack_shift_for(r.walks_per_point) : -1;
+  // packed units: a unit of fewer than 64 walks would hold one of the wave's
+  // 8 unit slots for a few lanes (85,120 one-walk units of plain walks:
+  // 10.9 ms with most lanes idle)
+  a.pack_shift = pack_shift_for(r.walks_per_point);
   if (a.pack_shift >= 0) {
     const int ppu = 64 >> a.pack_shift;
     a.total_units = (r.n_points + ppu - 1) / ppu;

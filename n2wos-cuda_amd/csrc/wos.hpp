@@ -66,8 +66,8 @@ struct WosKernelArgs {
   float* path_pos;
   float* path_q;
   int path_len;
-  // extended walker only: packed units (pack_shift >= 0, walks_per_point =
-  // 2^pack_shift <= 32): unit u holds the walks of points
+  // packed units (pack_shift >= 0, walks_per_point = 2^pack_shift <= 32):
+  // unit u holds the walks of points
   // [u 64 / wpp, (u + 1) 64 / wpp); unit arrays are indexed by point and the
   // step and flag counts are accumulated there (zeroed before the launch)
   int pack_shift;

@@ -229,15 +229,15 @@ extern "C" int emu_walker(const void* nodes, const void* tris, float delta, cons
   a.weight_update = w_update;
   a.n_points = n_points;
   a.pack_shift = -1;
+  // packed units for wpp = 1, 2, 4, ..., 32, as the host launch sets them up
+  for (int k = 0; k <= 5; ++k)
+    if (wpp == (1 << k)) a.pack_shift = k;
+  if (a.pack_shift >= 0) {
+    const int ppu = 64 >> a.pack_shift;
+    a.total_units = (n_points + ppu - 1) / ppu;
+    for (int i = 0; i < n_points; ++i) unit_steps[i] = unit_flags[i] = 0u;
+  }
   if (g_ext) {
-    // packed units for wpp = 1, 2, 4, ..., 32, as the host launch sets them up
-    for (int k = 0; k <= 5; ++k)
-      if (wpp == (1 << k)) a.pack_shift = k;
-    if (a.pack_shift >= 0) {
-      const int ppu = 64 >> a.pack_shift;
-      a.total_units = (n_points + ppu - 1) / ppu;
-      for (int i = 0; i < n_points; ++i) unit_steps[i] = unit_flags[i] = 0u;
-    }
     a.cv_values = g_ext_args.cv_values;
     a.cv_res = g_ext_args.cv_res;
     a.cv_points = g_ext_args.cv_points;

@@ -107,6 +107,14 @@ def test_emulated_walker_extreme_schedules(eng, orc, emu, weights, refill):
     _case(eng, orc, emu, "mesh1", 3, 64, False, weights=weights, refill=refill)
 
 
+@pytest.mark.parametrize("npts,wpp,poisson", [(37, 4, False), (70, 1, True), (5, 32, False), (9, 16, True)])
+def test_emulated_walker_packed_units(eng, orc, emu, npts, wpp, poisson):
+    """Plain walks with 1 to 32 walks per point run in packed units (64 / wpp
+    points per unit, the last one partial): the oracle's walks bit for bit,
+    and per-point statistics in the oracle's reduction order."""
+    _case(eng, orc, emu, "mesh1", npts, wpp, poisson)
+
+
 def test_emulated_walker_sah_tree(eng, orc, emu):
     """Walks over a surface-area-heuristic tree (leaf 4) equal the oracle's."""
     _case(eng, orc, emu, "mesh2", 5, 64, False, weights=(10, 10, 30), refill=2, eps=1e-4, build_method="sah", leaf=4)
