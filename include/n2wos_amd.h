@@ -142,7 +142,11 @@ int n2wos_bvh_get_stats(const n2wos_bvh* bvh, n2wos_bvh_stats* out);
  * index (lowest index among exact ties), overflow flag (0 unless the
  * traversal stack overflowed and the result may be inexact).  A point with a
  * non-finite coordinate yields d2 = FLT_MAX, closest = (0,0,0), tri = -1.
- * query_count == 0 is a no-op; < 0 -> N2WOS_ERR_INVALID. */
+ * query_count == 0 is a no-op; < 0 -> N2WOS_ERR_INVALID.
+ * A BVH handle is read-only after n2wos_bvh_create: queries and solves that
+ * use it may run concurrently on different streams and host threads (each
+ * with its own output buffers and workspace); n2wos_bvh_destroy must not
+ * overlap any of them. */
 int n2wos_bvh_query_device(const n2wos_bvh* bvh, const float* d_points, int query_count, float* d_distance2,
                            float* d_closest, int* d_triangle_id, int* d_overflow, int block_size,
                            void* stream);
