@@ -250,7 +250,7 @@ int n2wos_wos_solve(const n2wos_bvh* bvh, const n2wos_boundary* boundary, const 
 
 /* Largest walker workspace (n2wos_wos_workspace_size, bytes) that
  * n2wos_wos_solve_noalloc serves: 32 MiB.  For example 170,132 points x 256
- * walks need 20.4 MB; 680,000 points x 256 walks do not fit. */
+ * walks need 22.0 MB; 680,000 points x 256 walks do not fit. */
 #define N2WOS_NOALLOC_SCRATCH_BYTES (32LL << 20)
 /* n2wos_wos_solve without the workspace arguments, for callers that cannot
  * allocate device memory: code under HIP graph capture, tight serving loops,

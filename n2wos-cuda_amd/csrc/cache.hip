@@ -21,6 +21,7 @@
 #include "api_internal.hpp"
 #include "cache.hpp"
 #include "geom.hpp"
+#include "wave.hpp"
 
 typedef _Float16 h4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
@@ -273,10 +274,6 @@ __device__ __forceinline__ h4_t mask_relu(f4_t d, h4_t h) {
   r[3] = static_cast<_Float16>(static_cast<float>(h[3]) > 0.0f ? d[3] : 0.0f);
   return r;
 }
-
-// The XCD (accelerator complex die) this wave runs on: hardware register
-// XCC_ID (20) on gfx94x / gfx950, 0-7 on MI355X.
-__device__ __forceinline__ uint32_t xcd_id() { return __builtin_amdgcn_s_getreg((31 << 11) | (0 << 6) | 20); }
 
 // Grid gradients go to copy (xcd_id & copy_mask) of grid_grad (copies
 // gstride floats apart), summed by Adam: small grids get one copy per XCD so

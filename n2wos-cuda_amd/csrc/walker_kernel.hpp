@@ -136,6 +136,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
   // wave-uniform fill cursor
   int fill_slot = -1, fill_next = 0, fill_count = 0, fill_point = 0, fill_w0 = 0;
   bool no_more = false;
+  int ranges_done = 0;  // wave-uniform: unit ranges found exhausted
 
   // walk state
   int phase = kIdle;
@@ -296,13 +297,30 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
             }
           }
           if (fs < 0) break;
-          int u = 0;
-          if (lane == 0) u = atomicAdd(KA->next_unit, 1);
-          u = __shfl(u, 0);
-          if (u >= KA->total_units) {
+          // claim from this XCD's range first, then from the others in turn
+          int u = -1;
+          const int nr = KA->unit_ranges;
+          const int home = nr > 1 ? xcd_id() % nr : 0;
+          for (int k = 0; k < nr && u < 0; ++k) {
+            const int r = home + k < nr ? home + k : home + k - nr;
+            if ((ranges_done >> r) & 1) continue;
+            const int b0 = static_cast<int>(static_cast<int64_t>(r) * KA->total_units / nr);
+            const int b1 = static_cast<int>(static_cast<int64_t>(r + 1) * KA->total_units / nr);
+            int t = 0;
+            if (lane == 0) t = atomicAdd(KA->next_unit + r * kUnitCounterStride, 1);
+            t = __shfl(t, 0);
+            if (t < b1 - b0) {
+              u = b0 + t;
+            } else {
+              ranges_done |= 1 << r;
+            }
+          }
+          if (u < 0) {
             no_more = true;
             break;
           }
+          // pt: the unit's first position in the claim order (KA->order
+          // maps positions to points; identity when null)
           int pt, chunk, ucnt;
           if (KA->pack_shift >= 0) {
             // packed unit: 64 / wpp whole points
@@ -342,6 +360,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
             point = fill_point;
             walk = fill_w0 + wiu;
           }
+          if (KA->order) point = KA->order[point];
           if (KA->point_rng) {
             ctr_point = KA->point_rng[2 * point];
             ctr_walk = KA->point_rng[2 * point + 1] + static_cast<uint32_t>(walk);
@@ -559,16 +578,19 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
         }
         if (packed) {
           if ((lane & (span - 1)) == 0 && lane < ucnt) {
-            const int pp = pt + (lane >> KA->pack_shift);
+            int pp = pt + (lane >> KA->pack_shift);
+            if (KA->order) pp = KA->order[pp];
             KA->unit_mean[pp] = m;
             KA->unit_m2[pp] = q;
           }
           if (lane == 0) s_unit[wv][s] = -1;
         } else if (lane == 0) {
-          KA->unit_mean[u] = m;
-          KA->unit_m2[u] = q;
-          KA->unit_steps[u] = s_steps[wv][s];
-          KA->unit_flags[u] = (s_forced[wv][s] & 0xffffu) | (s_ovf[wv][s] << 16);
+          // the point's units are stored in walk order at point * units_per_point
+          const int uo = KA->order ? KA->order[pt] * KA->units_per_point + (u - pt * KA->units_per_point) : u;
+          KA->unit_mean[uo] = m;
+          KA->unit_m2[uo] = q;
+          KA->unit_steps[uo] = s_steps[wv][s];
+          KA->unit_flags[uo] = (s_forced[wv][s] & 0xffffu) | (s_ovf[wv][s] << 16);
           s_unit[wv][s] = -1;
         }
         wave_sync();

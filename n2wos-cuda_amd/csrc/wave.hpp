@@ -25,6 +25,12 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// The XCD (accelerator complex die) this wave runs on: hardware register
+// XCC_ID (20) on gfx94x / gfx950, 0-7 on MI355X.
+__device__ __forceinline__ int xcd_id() {
+  return static_cast<int>(__builtin_amdgcn_s_getreg((31 << 11) | (0 << 6) | 20));
+}
+
 // The hardware square root (V_SQRT_F32): one instruction, not correctly
 // rounded (within 1 ulp for normal inputs), against about 15 for sqrtf.
 // Only for bounds that carry a margin, never for values the results depend on.

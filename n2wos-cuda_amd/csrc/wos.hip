@@ -79,11 +79,79 @@ __global__ void k_point_merge(int n_points, int units_per_point, int walks_per_p
   out_forced[i] = forced;
 }
 
-__global__ void k_zero_counter(int* c) { *c = 0; }
+__global__ void k_zero_counters(int* c) {
+  if (threadIdx.x < kMaxUnitRanges) c[threadIdx.x * kUnitCounterStride] = 0;
+}
 
 __global__ void k_zero_u32(uint32_t* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
+}
+
+// ---------------------------------------------------------------- claim order
+// The walker claims units in the order of the points' cells on a 32^3 grid
+// over the mesh bounds, the cells taken in Morton (Z) order: walks that run at
+// the same time then start close together and touch the same mesh lines (on
+// the 1 M-triangle mesh of problem 3, 14-17% faster than random point order).
+// A counting sort: cells and counts, an exclusive scan of the counts, then a
+// scatter.  The order inside a cell follows the atomics and may differ from run
+// to run; results do not depend on it (each point's walks, statistics and
+// merge order are fixed by the point alone).
+__device__ __forceinline__ uint32_t spread3_5(uint32_t v) {
+  // 5 bits b4..b0 -> b4 at bit 12, ..., b0 at bit 0 (every third bit)
+  v &= 31u;
+  v = (v | (v << 8)) & 0x100Fu;
+  v = (v | (v << 4)) & 0x10C3u;
+  v = (v | (v << 2)) & 0x1249u;
+  return v;
+}
+
+__global__ void k_order_cells(const float* __restrict__ pts, int n, float lox, float loy, float loz, float sx,
+                              float sy, float sz, uint32_t* __restrict__ cell, uint32_t* __restrict__ count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  // fmaxf(NaN, 0) = 0: a non-finite coordinate lands in cell 0
+  const float fx = fminf(fmaxf((pts[3 * i] - lox) * sx, 0.0f), static_cast<float>(kOrderAxis - 1));
+  const float fy = fminf(fmaxf((pts[3 * i + 1] - loy) * sy, 0.0f), static_cast<float>(kOrderAxis - 1));
+  const float fz = fminf(fmaxf((pts[3 * i + 2] - loz) * sz, 0.0f), static_cast<float>(kOrderAxis - 1));
+  const uint32_t c = spread3_5(static_cast<uint32_t>(fx)) | (spread3_5(static_cast<uint32_t>(fy)) << 1) |
+                     (spread3_5(static_cast<uint32_t>(fz)) << 2);
+  cell[i] = c;
+  atomicAdd(count + c, 1u);
+}
+
+// One block of 1024 threads: exclusive prefix sum of the kOrderCells counts
+// into `cursor` (each thread scans 32 consecutive cells).
+__global__ void __launch_bounds__(1024) k_order_scan(const uint32_t* __restrict__ count,
+                                                     uint32_t* __restrict__ cursor) {
+  constexpr int kPer = kOrderCells / 1024;
+  __shared__ uint32_t part[1024];
+  const int t = static_cast<int>(threadIdx.x);
+  uint32_t sum = 0;
+  for (int k = 0; k < kPer; ++k) sum += count[t * kPer + k];
+  part[t] = sum;
+  __syncthreads();
+  // Hillis-Steele inclusive scan of the 1024 partial sums
+  for (int off = 1; off < 1024; off *= 2) {
+    const uint32_t add = t >= off ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += add;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - sum;
+  for (int k = 0; k < kPer; ++k) {
+    const uint32_t c = count[t * kPer + k];
+    cursor[t * kPer + k] = run;
+    run += c;
+  }
+}
+
+__global__ void k_order_scatter(const uint32_t* __restrict__ cell, int n, uint32_t* __restrict__ cursor,
+                                int* __restrict__ order) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t pos = atomicAdd(cursor + cell[i], 1u);
+  order[pos] = i;
 }
 
 // Packed units: walks_per_point a power of two <= 32 (log2 of it, else -1).
@@ -165,10 +233,12 @@ int wos_full_grid(int source, bool ext) {
 
 int64_t wos_workspace_bytes(int n_points, int walks_per_point) {
   const int64_t units = static_cast<int64_t>(n_points) * ((walks_per_point + 63) / 64);
-  // counter (256 B pad) + mean + m2 (8 B each) + steps + flags (4 B each)
+  // counters (256 B pad) + mean + m2 (8 B each) + steps + flags (4 B each)
   // + first query per point: d2, closest xyz, tri, overflow (24 B), 256-aligned
+  // + claim order: cell and order per point (8 B), cell counters (2 x 128 KiB)
   const int64_t first = (static_cast<int64_t>(n_points) * 24 + 255) / 256 * 256;
-  return 256 + (units * 24 + 255) / 256 * 256 + first;
+  const int64_t order = (static_cast<int64_t>(n_points) * 8 + 255) / 256 * 256 + 2 * 4 * kOrderCells;
+  return 256 + (units * 24 + 255) / 256 * 256 + first + order;
 }
 
 void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStream_t stream) {
@@ -216,8 +286,32 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
   a.first_d2 = first_d2;
   a.first_cp = first_cp;
   a.first_ovf = first_ovf;
+  char* ord = first + (static_cast<int64_t>(r.n_points) * 24 + 255) / 256 * 256;
+  uint32_t* order_cell = reinterpret_cast<uint32_t*>(ord);
+  int* order = reinterpret_cast<int*>(order_cell + r.n_points);
+  uint32_t* order_count = reinterpret_cast<uint32_t*>(ord + (static_cast<int64_t>(r.n_points) * 8 + 255) / 256 * 256);
+  uint32_t* order_cursor = order_count + kOrderCells;
+  a.order = nullptr;
+  if (r.n_points >= kOrderMinPoints && tuning_value("spatial_order", 1) != 0) {
+    // claim order by cell of the mesh bounds (the points themselves may lie outside)
+    float lo[3], sc[3];
+    for (int k = 0; k < 3; ++k) {
+      const float ext = bvh->bounds_hi[k] - bvh->bounds_lo[k];
+      lo[k] = bvh->bounds_lo[k];
+      sc[k] = ext > 0.0f ? static_cast<float>(kOrderAxis) / ext : 0.0f;
+    }
+    const int ob = 256;
+    const int og = (r.n_points + ob - 1) / ob;
+    hipLaunchKernelGGL(k_zero_u32, dim3(kOrderCells / ob), dim3(ob), 0, stream, order_count, kOrderCells);
+    hipLaunchKernelGGL(k_order_cells, dim3(og), dim3(ob), 0, stream, r.d_points, r.n_points, lo[0], lo[1], lo[2],
+                       sc[0], sc[1], sc[2], order_cell, order_count);
+    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, stream, order_count, order_cursor);
+    hipLaunchKernelGGL(k_order_scatter, dim3(og), dim3(ob), 0, stream, order_cell, r.n_points, order_cursor, order);
+    a.order = order;
+  }
 
-  hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, a.next_unit);
+  a.unit_ranges = std::max(1, std::min(kMaxUnitRanges, tuning_value("unit_ranges", kMaxUnitRanges)));
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, stream, a.next_unit);
   if (a.pack_shift >= 0) {
     // packed units accumulate step and flag counts per point
     const int zb = 256;

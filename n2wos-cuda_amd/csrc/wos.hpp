@@ -11,6 +11,19 @@
 
 namespace n2 {
 
+// Units are split into at most kMaxUnitRanges contiguous ranges, one per XCD:
+// a wave claims from its own XCD's range first (the points of a range share
+// the mesh lines their walks touch, which then stay in that XCD's L2), and
+// from the others once it is exhausted.  The counters are 32 bytes apart.
+constexpr int kMaxUnitRanges = 8;
+constexpr int kUnitCounterStride = 8;
+
+// Claim order (spatial): 32 cells per axis over the mesh bounds, Morton order;
+// solves of fewer points claim in the caller's order.
+constexpr int kOrderAxis = 32;
+constexpr int kOrderCells = kOrderAxis * kOrderAxis * kOrderAxis;
+constexpr int kOrderMinPoints = 8192;
+
 struct WosKernelArgs {
   const Bvh4Node* nodes;
   const TriRec* tris;
@@ -30,7 +43,14 @@ struct WosKernelArgs {
   // instead of point + point_offset and walk, so a point's walks are the
   // same whichever batch it runs in)
   const uint32_t* point_rng;
+  // unit claim counters: range r of unit_ranges contiguous unit ranges is
+  // claimed through next_unit[r * kUnitCounterStride]
   int* next_unit;
+  int unit_ranges;
+  // claim order: units are claimed in the order of positions, position i
+  // holding point order[i] (null: point i), so that the walks in flight at
+  // once start close together and share the mesh lines they touch
+  const int* order;
   double* unit_mean;
   double* unit_m2;
   uint32_t* unit_steps;

@@ -113,6 +113,11 @@ inline int rank_in(uint64_t mask) {
   return l == 0 ? 0 : __builtin_popcountll(mask & ((~0ull) >> (64 - l)));
 }
 inline void wave_sync() { emu::arrive(emu::kSync); }
+// the emulated wave's XCD: set through emu_set_unit_ranges
+inline int emu_xcd = 3;
+inline int emu_ranges = 8;
+inline const int* emu_order = nullptr;
+inline int xcd_id() { return emu_xcd; }
 }  // namespace n2
 // the emulated block is one wave
 inline void __syncthreads() { n2::wave_sync(); }
@@ -210,9 +215,11 @@ extern "C" int emu_walker(const void* nodes, const void* tris, float delta, cons
   a.seed_hi = static_cast<uint32_t>(seed >> 32);
   a.stream = 0;
   a.point_offset = 0;
-  static int next_unit;
-  next_unit = 0;
-  a.next_unit = &next_unit;
+  static int next_unit[n2::kMaxUnitRanges * n2::kUnitCounterStride];
+  for (int& c : next_unit) c = 0;
+  a.next_unit = next_unit;
+  a.unit_ranges = n2::emu_ranges;
+  a.order = n2::emu_order;
   a.unit_mean = unit_mean;
   a.unit_m2 = unit_m2;
   a.unit_steps = unit_steps;
@@ -282,6 +289,15 @@ extern "C" void emu_set_ext(const float* values, int res, int points, const floa
 extern "C" long emu_iterations() { return emu::iters; }
 // Sets the triangle array's size for the load bounds check (-1: off) and
 // clears the count of loads outside it.
+// Unit ranges of the claim (1-8) and the XCD the emulated wave reports.
+extern "C" void emu_set_unit_ranges(int ranges, int xcd) {
+  n2::emu_ranges = ranges;
+  n2::emu_xcd = xcd;
+}
+
+// Claim order: positions -> points (NULL: identity).
+extern "C" void emu_set_order(const int* order) { n2::emu_order = order; }
+
 extern "C" void emu_set_tri_count(long n) {
   emu::tri_count = n;
   emu::tri_oob = 0;

@@ -170,6 +170,43 @@ def test_partial_grid_equals_full_grid(eng, torch_gpu, m1):
         assert np.array_equal(a.steps.cpu().numpy(), b.steps.cpu().numpy())
 
 
+@pytest.mark.parametrize("npts,wpp", [(12000, 16), (9000, 70)])
+def test_claim_order_and_ranges_change_nothing(eng, orc, torch_gpu, m1, npts, wpp):
+    """Solves of at least 8,192 points claim their units in spatial (Morton
+    cell) order, split into per-XCD ranges.  Per-walk values, steps and FP64
+    statistics equal those of the caller's order with one range bit for bit,
+    and a sample of points equals the oracle's walks."""
+    from n2wos_amd import _lib as L, problems as P, wos as W
+    torch = torch_gpu
+    v, t, bvh, b2 = m1
+    pts = P.interior_points(bvh, v, npts, seed=21)
+    bc = P.charges_boundary(0)
+    wp = W.WalkParams(epsilon=1e-3, seed=8)
+    print("case: claim order %dx%d" % (npts, wpp), flush=True)
+    runs = []
+    try:
+        for order, ranges in [(1, 8), (0, 1), (1, 1), (0, 8)]:
+            L.check(L.lib().n2wos_set_tuning(b"spatial_order", order))
+            L.check(L.lib().n2wos_set_tuning(b"unit_ranges", ranges))
+            runs.append(run(torch, bvh, bc, wp, pts, wpp))
+    finally:
+        L.check(L.lib().n2wos_set_tuning(b"spatial_order", 1))
+        L.check(L.lib().n2wos_set_tuning(b"unit_ranges", 8))
+    a = runs[0]
+    for b in runs[1:]:
+        assert np.array_equal(bits(a.extra["walk_values"].cpu().numpy()), bits(b.extra["walk_values"].cpu().numpy()))
+        assert np.array_equal(a.extra["walk_steps"].cpu().numpy(), b.extra["walk_steps"].cpu().numpy())
+        for f in ("n", "mean", "m2"):
+            assert np.array_equal(getattr(a, f).cpu().numpy().view(np.uint64), getattr(b, f).cpu().numpy().view(np.uint64))
+        assert np.array_equal(a.steps.cpu().numpy(), b.steps.cpu().numpy())
+        assert np.array_equal(a.forced.cpu().numpy(), b.forced.cpu().numpy())
+    sel = np.arange(0, npts, npts // 37)[:37]
+    ov, os_, _, _ = b2.walks(pts[sel], wpp, max_steps=256, eps=1e-3, step_scale=0.999, bmode=bc.mode,
+                             bparams=bc.oracle_params(), seed=8, nthreads=16, point_ids=sel)
+    gv = a.extra["walk_values"].cpu().numpy().reshape(npts, wpp)[sel].ravel()
+    assert np.array_equal(bits(gv), bits(ov))
+
+
 @pytest.mark.parametrize("mesh", ["sphere8x16", "mesh1"])
 def test_exterior_start_points_bit_exact(eng, orc, torch_gpu, mesh):
     """Walks from points outside the closed mesh drift away until the step cap

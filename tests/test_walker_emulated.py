@@ -39,6 +39,8 @@ def emu():
                              vp, vp, i, i, i, i, C.c_long]
     E.emu_walker.restype = i
     E.emu_set_tri_count.argtypes = [C.c_long]
+    E.emu_set_unit_ranges.argtypes = [i, i]
+    E.emu_set_order.argtypes = [vp]
     E.emu_tri_oob.restype = C.c_long
     E.emu_set_ext.argtypes = [vp, i, i, vp, vp, f, vp, vp, i]
     return E
@@ -113,6 +115,33 @@ def test_emulated_walker_packed_units(eng, orc, emu, npts, wpp, poisson):
     points per unit, the last one partial): the oracle's walks bit for bit,
     and per-point statistics in the oracle's reduction order."""
     _case(eng, orc, emu, "mesh1", npts, wpp, poisson)
+
+
+@pytest.mark.parametrize("ranges,xcd,npts,wpp", [(1, 0, 20, 64), (8, 0, 20, 64), (8, 7, 20, 64), (3, 2, 11, 100),
+                                                 (8, 5, 3, 64), (8, 6, 37, 4)])
+def test_emulated_walker_unit_ranges(eng, orc, emu, ranges, xcd, npts, wpp):
+    """Units claimed range by range (the wave's own XCD's range first, then
+    the others in turn; ranges may be empty) are each run exactly once: the
+    oracle's walks and statistics bit for bit."""
+    emu.emu_set_unit_ranges(ranges, xcd)
+    try:
+        _case(eng, orc, emu, "mesh1", npts, wpp, False)
+    finally:
+        emu.emu_set_unit_ranges(8, 3)
+
+
+@pytest.mark.parametrize("npts,wpp,ranges", [(37, 4, 8), (11, 100, 3), (70, 1, 8), (20, 64, 1)])
+def test_emulated_walker_claim_order(eng, orc, emu, npts, wpp, ranges):
+    """Units claimed through a permutation of the points (the spatial claim
+    order) give each point the oracle's walks and statistics bit for bit."""
+    order = np.random.default_rng(npts + wpp).permutation(npts).astype(np.int32)
+    emu.emu_set_order(order.ctypes.data)
+    emu.emu_set_unit_ranges(ranges, 1)
+    try:
+        _case(eng, orc, emu, "mesh1", npts, wpp, False)
+    finally:
+        emu.emu_set_order(None)
+        emu.emu_set_unit_ranges(8, 3)
 
 
 def test_emulated_walker_sah_tree(eng, orc, emu):
