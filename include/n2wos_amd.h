@@ -72,7 +72,12 @@ int n2wos_abi_version(void);
  * that trigger a refill (1..64, default 2); "weight_node" / "weight_tri" /
  * "weight_update" = per-lane weights of the walker's phase choice (10, 10 and
  * 30); "query_quad" = 1 (default): closest-point queries run on four lanes
- * each, 0: one lane per query with the caller's block size (same results). */
+ * each, 0: one lane per query with the caller's block size (same results);
+ * "spatial_order" = 1 (default): solves of at least 8,192 points claim their
+ * work in Morton order of 32^3 cells over the mesh bounds, 0: in the
+ * caller's point order; "unit_ranges" = 1..8 (default 8): the claim order is
+ * split into that many ranges, a wave claiming from its XCD's range first.
+ * Neither changes any result. */
 int n2wos_set_tuning(const char* key, int value);
 /* Device name and runtime version of the current HIP device, for reports. */
 int n2wos_device_summary(char* buf, int buf_len);
