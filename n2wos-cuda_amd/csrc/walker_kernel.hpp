@@ -96,6 +96,10 @@ constexpr int kNode = 1;    // query in progress, at an inner node
 constexpr int kTri = 2;     // query in progress, testing the triangles of a leaf
 constexpr int kUpdate = 3;  // query finished: take the walk step (or end the walk)
 
+// Records of its leaf a lane offers to one cooperative triangle iteration
+// (a leaf of more is finished in later iterations).
+constexpr int kTriTake = 4;
+
 // Every iteration the wave serves ONE phase, chosen by ballot (the phase with
 // the most lanes per instruction of its path), so divergent lanes never pay
 // for paths they do not take, and a lane whose query ends starts its next
@@ -117,6 +121,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
   __shared__ uint32_t s_forced[kWaves][S];
   __shared__ uint32_t s_ovf[kWaves][S];
   __shared__ float s_charge[4 * kMaxCharges];
+  __shared__ uint32_t s_task[kWaves][64];  // triangle tasks: record << 6 | owner lane
 
   // arguments used only on rare paths (refill, walk step, walk end) are read
   // where they are used instead of being held in scalar registers across the
@@ -253,32 +258,6 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
       N2_PUSH3(nvalid - 1, r1, k1, r2, k2, r3, k3);                                \
       N2_ENTER(r0);                                                                \
     }                                                                              \
-  } while (0)
-  // one triangle record (a, b, c, original index) of the current leaf.  The
-  // exact test runs for every record: the wave is VALU-bound, and a per-lane
-  // box pre-test saves nothing unless every served lane rejects.
-#define N2_TRI_TEST(t0, t1, t2)                                                    \
-  do {                                                                             \
-    if (COUNT) ++n_tris;                                                           \
-    const V3 ta = v3(t0.x, t0.y, t0.z), tb = v3(t0.w, t1.x, t1.y), tc = v3(t1.z, t1.w, t2.x); \
-    const int id = __float_as_int(t2.y);                                           \
-    const V3 q = closest_on_triangle_flat(p, ta, tb, tc);                          \
-    const V3 e = q - p;                                                            \
-    const float d2 = dot(e, e);                                                    \
-    /* selects instead of a branch: the registers are updated in place */       \
-    /* bitwise, not short-circuit: no branch.  pt_ is used only when d2 is  */ \
-    /* better, so d2 is not NaN; the threshold can be NaN only for a NaN     */ \
-    /* point (vertices are finite), whose d2 is NaN: the select is fminf's   */ \
-    /* result wherever it is used */                                            \
-    const bool better_ = (d2 < bd2) | ((d2 == bd2) & (id < bid));                  \
-    const float ptc_ = walker_prune_threshold(d2, a.delta);                        \
-    const float pt_ = ptc_ < thr ? ptc_ : thr;                                     \
-    bd2 = better_ ? d2 : bd2;                                                      \
-    bcp = v3(better_ ? q.x : bcp.x, better_ ? q.y : bcp.y, better_ ? q.z : bcp.z); \
-    bid = better_ ? id : bid;                                                      \
-    thr = better_ ? pt_ : thr;                                                     \
-    ++cur;                                                                         \
-    if (cur >= te) N2_POP_NEXT();                                                   \
   } while (0)
 
   while (true) {
@@ -603,12 +582,67 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
         N2_NODE_VISIT(n0, n1, n2, n3, n4, n5, n6);
       }
     } else {
-      // ------------------------------------------------------ one triangle
+      // ------------------------------------- triangle tests, wave-cooperative
+      // Each lane in the triangle phase offers the next (up to kTriTake)
+      // records of its leaf as tasks; the first 64 tasks in lane order are
+      // spread one per lane and tested at once, so a wave with 24 lanes in
+      // the phase tests about 60 records per iteration instead of 24.  The
+      // owner of a task then folds the results into its best in record order,
+      // exactly as testing them one by one would (every record of a leaf is
+      // tested: no pre-test).
+      const int offer = phase == kTri ? max(0, min(te - cur, kTriTake)) : 0;
+      const uint64_t tb0 = __ballot(offer & 1), tb1 = __ballot(offer & 2), tb2 = __ballot(offer & 4);
+      const int pre = rank_in(tb0) + 2 * rank_in(tb1) + 4 * rank_in(tb2);  // first task of this lane
+      const int ntask = min(__popcll(tb0) + 2 * __popcll(tb1) + 4 * __popcll(tb2), 64);
+      const int take = max(0, min(offer, 64 - pre));  // this lane's tasks served now
+      for (int j = 0; j < take; ++j) s_task[wv][pre + j] = (static_cast<uint32_t>(cur + j) << 6) | lane;
+      wave_sync();
+      const bool tasked = lane < ntask;
+      const uint32_t task = tasked ? s_task[wv][lane] : static_cast<uint32_t>(lane);
+      const int owner = static_cast<int>(task & 63u);
+      const V3 tp = v3(lane_read_f(p.x, owner), lane_read_f(p.y, owner), lane_read_f(p.z, owner));
+      float td2 = 0.0f, tpt = 0.0f;
+      int tid = 0;
+      V3 tq = v3(0.0f, 0.0f, 0.0f);
+      if (tasked) {
+        int rec = static_cast<int>(task >> 6);
+        N2_EMU_TRI_HOOK(rec);
+        const float4* trp = reinterpret_cast<const float4*>(a.tris + rec);
+        const float4 t0 = trp[0], t1 = trp[1], t2 = trp[2];
+        const V3 ta = v3(t0.x, t0.y, t0.z), tb = v3(t0.w, t1.x, t1.y), tc = v3(t1.z, t1.w, t2.x);
+        tid = __float_as_int(t2.y);
+        tq = closest_on_triangle_flat(tp, ta, tb, tc);
+        const V3 e = tq - tp;
+        td2 = dot(e, e);
+        tpt = walker_prune_threshold(td2, a.delta);
+      }
+      // fold in record order.  Selects, no branches: pt is used only when d2
+      // is better, so d2 is not NaN; the threshold can be NaN only for a NaN
+      // point (vertices are finite), whose d2 is NaN: the select is fminf's
+      // result wherever it is used
+      int bj = -1;
+      for (int j = 0; j < kTriTake; ++j) {
+        const bool has = j < take;
+        if (__ballot(has) == 0) break;
+        const int src = has ? pre + j : lane;
+        const float d2 = lane_read_f(td2, src);
+        const int id = static_cast<int>(lane_read(static_cast<uint32_t>(tid), src));
+        const float ptc = lane_read_f(tpt, src);
+        const bool better = has & ((d2 < bd2) | ((d2 == bd2) & (id < bid)));
+        const float pt = ptc < thr ? ptc : thr;
+        bd2 = better ? d2 : bd2;
+        bid = better ? id : bid;
+        thr = better ? pt : thr;
+        bj = better ? j : bj;
+      }
+      // the closest point of the last (so the best) improvement
+      const int qs = bj >= 0 ? pre + bj : lane;
+      const V3 q = v3(lane_read_f(tq.x, qs), lane_read_f(tq.y, qs), lane_read_f(tq.z, qs));
+      if (bj >= 0) bcp = q;
       if (phase == kTri) {
-        N2_EMU_TRI_HOOK(cur);
-        const float4* tp = reinterpret_cast<const float4*>(a.tris + cur);
-        const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-        N2_TRI_TEST(t0, t1, t2);
+        if (COUNT) n_tris += static_cast<uint32_t>(take);
+        cur += take;
+        if (cur >= te) N2_POP_NEXT();
       }
     }
   }
@@ -617,7 +651,6 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
 #undef N2_POP_NEXT
 #undef N2_PUSH3
 #undef N2_NODE_VISIT
-#undef N2_TRI_TEST
   if (COUNT) {
     atomicAdd(reinterpret_cast<unsigned long long*>(a.visits), static_cast<unsigned long long>(n_nodes));
     atomicAdd(reinterpret_cast<unsigned long long*>(a.visits) + 1, static_cast<unsigned long long>(n_tris));

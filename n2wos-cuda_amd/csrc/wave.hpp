@@ -20,6 +20,12 @@ __device__ __forceinline__ int rank_in(uint64_t mask) {
                                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u)));
 }
 
+// v of lane `src` (0-63) of this wave, one ds_bpermute: the whole wave
+// executes it, and src names an active lane
+__device__ __forceinline__ uint32_t lane_read(uint32_t v, int src) {
+  return static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(src << 2, static_cast<int>(v)));
+}
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -60,5 +66,9 @@ __device__ __forceinline__ kernarg_ptr<T> kernel_args() {
   return p;
 }
 #endif
+
+__device__ __forceinline__ float lane_read_f(float v, int src) {
+  return __uint_as_float(lane_read(__float_as_uint(v), src));
+}
 
 }  // namespace n2

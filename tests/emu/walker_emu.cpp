@@ -85,13 +85,12 @@ inline double __shfl_down(double v, int dlt) {
     break;                                                                                         \
   }
 // Bounds check of every triangle-record load against the array's size.  A
-// load outside it is counted, then served from record 0 with the leaf ended
-// there, so the failure is reported instead of reading past the array.
+// load outside it is counted, then served from record 0, so the failure is
+// reported instead of reading past the array.
 #define N2_EMU_TRI_HOOK(idx)                                                                       \
   if (emu::tri_count >= 0 && ((idx) < 0 || (idx) >= emu::tri_count)) {                             \
     ++emu::tri_oob;                                                                                \
-    cur = 0;                                                                                       \
-    te = 1;                                                                                        \
+    (idx) = 0;                                                                                     \
   }
 #define N2_EMU_TRACE(sN, sT, sU)                                                                   \
   if (lane == 0) {                                                                                 \
@@ -113,6 +112,7 @@ inline int rank_in(uint64_t mask) {
   return l == 0 ? 0 : __builtin_popcountll(mask & ((~0ull) >> (64 - l)));
 }
 inline void wave_sync() { emu::arrive(emu::kSync); }
+inline uint32_t lane_read(uint32_t v, int src) { return static_cast<uint32_t>(__shfl(static_cast<int>(v), src)); }
 // the emulated wave's XCD: set through emu_set_unit_ranges
 inline int emu_xcd = 3;
 inline int emu_ranges = 8;
