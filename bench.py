@@ -66,10 +66,11 @@ METRIC = "walk-steps/s + time-to-RMSE≤1e-2 (NC+2LMC vs plain WoS); p99 req lat
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES = 112   # four child boxes (96 B) + four child refs (16 B)
 TRI_BYTES = 40     # three vertices (36 B) + original index (4 B)
-# tree of the walker workloads: SAH, 3 triangles per leaf (measured fastest
-# of leaf sizes 1-8 for the walker on mesh 2; results are identical for any
-# tree, since every query is exact)
-WALKER_TREE = dict(leaf_size=3, build_method="sah")
+# tree of the walker workloads: SAH, 4 triangles per leaf (measured fastest
+# of leaf sizes 2-8 for the walker, whose triangle tests are spread over the
+# wave, on mesh 2; results are identical for any tree, since every query is
+# exact)
+WALKER_TREE = dict(leaf_size=4, build_method="sah")
 
 
 def parse():
@@ -463,7 +464,7 @@ def main():
         "config": {
             "workload": "problem 2: plain GPU WoS, mesh2 (UV sphere 128x256, r=1+0.1 sin6t sin6p, 65,024 tris), "
                         f"{args.grid}x{args.grid} z=0 grid interior points, {args.wpp} walks/point, eps 1e-4, "
-                        "8-charge Laplace, FP64 per-point Welford/Chan, SAH BVH leaf 3",
+                        "8-charge Laplace, FP64 per-point Welford/Chan, SAH BVH leaf 4",
             "points": n,
             "walks_per_point": args.wpp,
             "triangles": int(len(tris)),

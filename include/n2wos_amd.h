@@ -231,7 +231,10 @@ typedef struct {
   void* event_end;
 } n2wos_walk_params;
 
-/* Bytes of device workspace n2wos_wos_solve needs for this job. */
+/* Bytes of device workspace n2wos_wos_solve needs for this job on the current
+ * device (a negative error code if the device cannot be queried): per-unit
+ * and per-point state plus a staging area for walk values whose size follows
+ * the device's resident walker blocks (at most 10.5 MB on MI355X). */
 int64_t n2wos_wos_workspace_size(int n_points, int walks_per_point);
 /* Blocks of the persistent walker's full-device grid on the current device
  * for this source term (the launch that max_blocks = 0 asks for), so that a
@@ -255,7 +258,7 @@ int n2wos_wos_solve(const n2wos_bvh* bvh, const n2wos_boundary* boundary, const 
 
 /* Largest walker workspace (n2wos_wos_workspace_size, bytes) that
  * n2wos_wos_solve_noalloc serves: 32 MiB.  For example 170,132 points x 256
- * walks need 22.0 MB; 680,000 points x 256 walks do not fit. */
+ * walks need 32.5 MB on MI355X; 340,000 points x 256 walks do not fit. */
 #define N2WOS_NOALLOC_SCRATCH_BYTES (32LL << 20)
 /* n2wos_wos_solve without the workspace arguments, for callers that cannot
  * allocate device memory: code under HIP graph capture, tight serving loops,

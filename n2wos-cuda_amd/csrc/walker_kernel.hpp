@@ -76,8 +76,8 @@ __host__ __device__ __forceinline__ void chan_merge(double na, double ma, double
 constexpr int kWaves = N2_WAVES_PER_BLOCK;  // waves per block
 constexpr int kBlock = 64 * kWaves;
 // Waves per SIMD the register allocation must allow: 5 (at most 96 VGPRs,
-// no spills).  The LDS ring of 8 entries per lane (6.3 KB per one-wave
-// block) leaves room for more; a ring of 16 (10.4 KB) allows about 4.
+// no spills).  A one-wave block holds 7.2 KB of LDS with a ring of 16
+// entries per lane (6 bytes each), so 20 blocks fit a CU's 160 KB.
 #ifndef N2_WALKER_WAVES_PER_SIMD
 #define N2_WALKER_WAVES_PER_SIMD 5
 #endif
@@ -100,6 +100,10 @@ constexpr int kUpdate = 3;  // query finished: take the walk step (or end the wa
 // (a leaf of more is finished in later iterations).
 constexpr int kTriTake = 4;
 
+// Walk flags beside the unit slot and walk-in-unit index (bits 0-8).
+constexpr int kFlagBounded = 1 << 9;  // the query started from the step bound, not unbounded
+constexpr int kFlagOvf = 1 << 10;     // the query's traversal stack overflowed
+
 // Every iteration the wave serves ONE phase, chosen by ballot (the phase with
 // the most lanes per instruction of its path), so divergent lanes never pay
 // for paths they do not take, and a lane whose query ends starts its next
@@ -111,10 +115,14 @@ constexpr int kTriTake = 4;
 // instantiation carries none of it.
 template <int CAP, int S, bool POISSON, bool COUNT, bool EXT = false>
 __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persistent(WosKernelArgs a) {
+  static_assert(S <= 8, "the unit slot is kept in 3 bits");
   // CAP ring slots per lane plus one spare slot that takes the writes of
-  // absent entries, so a node visit pushes without branches
-  __shared__ uint2 s_ent[(CAP + 1) * kBlock];
-  __shared__ float s_val[kWaves][S][64];
+  // absent entries, so a node visit pushes without branches.  An entry is a
+  // node ref and the high 16 bits of its squared box distance: the key
+  // truncated toward zero, a lower bound, so an entry is popped whenever its
+  // exact key would be (a few more, never fewer), and the search stays exact.
+  __shared__ uint32_t s_ref[(CAP + 1) * kBlock];
+  __shared__ uint16_t s_key[(CAP + 1) * kBlock];
   __shared__ int s_unit[kWaves][S];
   __shared__ int s_rem[kWaves][S];
   __shared__ uint32_t s_steps[kWaves][S];
@@ -133,7 +141,8 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
 #endif
   const int lane = lane_id();
   const int wv = static_cast<int>(threadIdx.x) >> 6;
-  uint2* sent = s_ent + threadIdx.x;
+  uint32_t* sref = s_ref + threadIdx.x;
+  uint16_t* skey = s_key + threadIdx.x;
   if (lane < S) s_unit[wv][lane] = -1;
   for (int i = static_cast<int>(threadIdx.x); i < 4 * kMaxCharges; i += kBlock) s_charge[i] = a.bc.charge[i >> 2][i & 3];
   __syncthreads();
@@ -145,17 +154,19 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
 
   // walk state
   int phase = kIdle;
-  int slot = 0, wiu = 0, point = 0, walk = 0, step = 0;
-  uint32_t ctr_point = 0, ctr_walk = 0;  // Philox counter words of the walk
+  // the walk's index in its unit (bits 0-5), its unit slot (6-8) and the
+  // flags kFlagBounded and kFlagOvf, in one register: all are used only on
+  // the walk-step and walk-end paths
+  int wflags = 0;
+  int point = 0, walk = 0, step = 0;
   V3 p = v3(0.0f, 0.0f, 0.0f);
   float src = 0.0f;
   float cvs = 0.0f;  // extended walker: sum of the control-variate terms
-  float thr0 = FLT_MAX;
   // query state
   // one position register: the node index in the node phase, the next
   // triangle record in the triangle phase (a lane is in one at a time)
   int cur = 0;
-  int te = 0, top = 0, cnt = 0, scnt = 0, bid = -1, ovf = 0;
+  int te = 0, top = 0, cnt = 0, scnt = 0, bid = -1;
   float thr = FLT_MAX, bd2 = FLT_MAX;
   V3 bcp = v3(0.0f, 0.0f, 0.0f);
   uint64_t spill_e[kSpill];
@@ -185,9 +196,8 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
       if (cnt > 0) {                                 \
         top = (top - 1) & (CAP - 1);                 \
         --cnt;                                       \
-        const uint2 le_ = sent[top * kBlock];        \
-        pop_key_ = __uint_as_float(le_.y);           \
-        pop_ref_ = le_.x;                            \
+        pop_key_ = __uint_as_float(static_cast<uint32_t>(skey[top * kBlock]) << 16); \
+        pop_ref_ = sref[top * kBlock];               \
       } else {                                       \
         --scnt;                                      \
         const uint64_t se_ = spill_e[scnt];          \
@@ -212,20 +222,23 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
     if (cnt + m_ > CAP) {                                                         \
       for (int j_ = CAP - cnt; j_ < m_; ++j_) {                                   \
         if (scnt < kSpill) {                                                      \
-          const uint2 ve_ = sent[((top + j_) & (CAP - 1)) * kBlock];              \
-          spill_e[scnt] = (static_cast<uint64_t>(ve_.y) << 32) | ve_.x;           \
+          const int vs_ = ((top + j_) & (CAP - 1)) * kBlock;                       \
+          spill_e[scnt] = (static_cast<uint64_t>(skey[vs_]) << 48) | sref[vs_];    \
           ++scnt;                                                                 \
         } else {                                                                  \
-          ovf = 1;                                                                \
+          wflags |= kFlagOvf;                                                     \
         }                                                                         \
       }                                                                           \
     }                                                                             \
     const int s1_ = m_ >= 1 ? ((top + m_ - 1) & (CAP - 1)) : CAP;                 \
     const int s2_ = m_ >= 2 ? ((top + m_ - 2) & (CAP - 1)) : CAP;                 \
     const int s3_ = m_ >= 3 ? top : CAP;                                          \
-    sent[s3_ * kBlock] = make_uint2((R3), __float_as_uint(K3));                   \
-    sent[s2_ * kBlock] = make_uint2((R2), __float_as_uint(K2));                   \
-    sent[s1_ * kBlock] = make_uint2((R1), __float_as_uint(K1));                   \
+    sref[s3_ * kBlock] = (R3);                                                    \
+    skey[s3_ * kBlock] = static_cast<uint16_t>(__float_as_uint(K3) >> 16);        \
+    sref[s2_ * kBlock] = (R2);                                                    \
+    skey[s2_ * kBlock] = static_cast<uint16_t>(__float_as_uint(K2) >> 16);        \
+    sref[s1_ * kBlock] = (R1);                                                    \
+    skey[s1_ * kBlock] = static_cast<uint16_t>(__float_as_uint(K1) >> 16);        \
     top = (top + m_) & (CAP - 1);                                                 \
     cnt = cnt + m_ < CAP ? cnt + m_ : CAP;                                        \
   } while (0)
@@ -330,8 +343,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
         const int r = rank_in(idle);
         if (is_idle && r < take) {
           // every walk of a point shares its first query: computed once per point
-          slot = fill_slot;
-          wiu = fill_next + r;
+          const int wiu = fill_next + r;
           if (KA->pack_shift >= 0) {
             point = fill_point + (wiu >> KA->pack_shift);
             walk = wiu & ((1 << KA->pack_shift) - 1);
@@ -340,22 +352,15 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
             walk = fill_w0 + wiu;
           }
           if (KA->order) point = KA->order[point];
-          if (KA->point_rng) {
-            ctr_point = KA->point_rng[2 * point];
-            ctr_walk = KA->point_rng[2 * point + 1] + static_cast<uint32_t>(walk);
-          } else {
-            ctr_point = static_cast<uint32_t>(point) + KA->point_offset;
-            ctr_walk = static_cast<uint32_t>(walk);
-          }
           step = 0;
           src = 0.0f;
           if (EXT) cvs = 0.0f;
-          thr0 = FLT_MAX;
           p = v3(KA->points[3 * point], KA->points[3 * point + 1], KA->points[3 * point + 2]);
           bd2 = KA->first_d2[point];
           bcp = v3(KA->first_cp[3 * point], KA->first_cp[3 * point + 1], KA->first_cp[3 * point + 2]);
           bid = 0;
-          ovf = KA->first_ovf[point];
+          // the first query is unbounded (kFlagBounded clear)
+          wflags = (fill_slot << 6) | wiu | (KA->first_ovf[point] ? kFlagOvf : 0);
           phase = kUpdate;
         }
         fill_next += take;
@@ -379,9 +384,9 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
       // ------------------------------------------------------ walk step
       bool unit_done = false;
       if (phase == kUpdate) {
-        if (bid < 0 && thr0 < FLT_MAX && finite3(p)) {
+        if (bid < 0 && (wflags & kFlagBounded) && finite3(p)) {
           // the step bound was too tight (cannot happen in exact arithmetic): redo unbounded
-          thr0 = FLT_MAX;
+          wflags &= ~kFlagBounded;
           thr = FLT_MAX;
           top = cnt = scnt = 0;
           cur = 0;
@@ -433,7 +438,11 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
                 KA->nc_xm[3 * sidx + 2] = p.z;
               }
             }
-            s_val[wv][slot][wiu] = value;
+            // staged per unit slot in global memory, indexed by walk-in-unit
+            const int slot = (wflags >> 6) & 7;
+            const bool ovf = (wflags & kFlagOvf) != 0;
+            KA->unit_vals[((static_cast<int64_t>(blockIdx.x) * kWaves + wv) * S + slot) * 64 + (wflags & 63)] =
+                value;
             if (KA->pack_shift >= 0) {
               // packed unit: the point's own counters (integer sums, any order)
               atomicAdd(&KA->unit_steps[point], static_cast<uint32_t>(step));
@@ -465,6 +474,16 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
             // antithetic pairs: the odd walk of a pair takes its first step
             // along the negated direction drawn for its even partner
             const bool anti = KA->antithetic && step == 0 && (walk & 1);
+            // the walk's Philox counter words, derived here rather than held
+            // in registers across the loop
+            uint32_t ctr_point, ctr_walk;
+            if (KA->point_rng) {
+              ctr_point = KA->point_rng[2 * point];
+              ctr_walk = KA->point_rng[2 * point + 1] + static_cast<uint32_t>(walk);
+            } else {
+              ctr_point = static_cast<uint32_t>(point) + KA->point_offset;
+              ctr_walk = static_cast<uint32_t>(walk);
+            }
             const uint32_t cw = anti ? ctr_walk - 1u : ctr_walk;
             const float sgn = anti ? -1.0f : 1.0f;
             // the key words are read here, so the compiler derives the ten
@@ -507,11 +526,12 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
             // clamped to FLT_MAX: a bound that overflows is the unbounded
             // query, and an empty child slot (box key +inf) stays unvisited
             const float tb = hw_sqrt(dot(eo, eo)) * 1.0000152587890625f + 4.0f * a.delta;
-            thr0 = fminf(tb * tb, FLT_MAX);
+            const float thr0 = fminf(tb * tb, FLT_MAX);
             bd2 = FLT_MAX;
             bcp = v3(0.0f, 0.0f, 0.0f);
             bid = -1;
-            ovf = 0;
+            // a new query: the overflow flag cleared, bounded unless thr0 is FLT_MAX
+            wflags = (wflags & 511) | (thr0 < FLT_MAX ? kFlagBounded : 0);
             if (finite3(p)) {
               thr = thr0;
               top = cnt = scnt = 0;
@@ -526,10 +546,12 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
       wave_sync();
       // ---------------------------------------------------- finish units
       uint64_t done = __ballot(unit_done);
+      // the unit values this wave stored are read back below
+      if (done != 0) wave_global_sync();
       while (done != 0) {
         const int l = __ffsll(static_cast<unsigned long long>(done)) - 1;
         done &= done - 1;
-        const int s = __shfl(slot, l);
+        const int s = (__shfl(wflags, l) >> 6) & 7;
         const int u = s_unit[wv][s];
         const bool packed = KA->pack_shift >= 0;
         int pt, ucnt, span;
@@ -543,7 +565,8 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
           span = 64;
         }
         double n = lane < ucnt ? 1.0 : 0.0;
-        double m = lane < ucnt ? static_cast<double>(s_val[wv][s][lane]) : 0.0;
+        const float* uv = KA->unit_vals + ((static_cast<int64_t>(blockIdx.x) * kWaves + wv) * S + s) * 64;
+        double m = lane < ucnt ? static_cast<double>(load_l2(uv + lane)) : 0.0;
         double q = 0.0;
         // Chan tree over lane pairs at strides 1, 2, 4, ...: within a packed
         // unit's segments it gives each point exactly what the full tree
@@ -587,13 +610,14 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
       // records of its leaf as tasks; the first 64 tasks in lane order are
       // spread one per lane and tested at once, so a wave with 24 lanes in
       // the phase tests about 60 records per iteration instead of 24.  The
-      // owner of a task then folds the results into its best in record order,
-      // exactly as testing them one by one would (every record of a leaf is
+      // owner of a task then folds the results into its best, which becomes
+      // what testing them one by one gives (every record of a leaf is
       // tested: no pre-test).
       const int offer = phase == kTri ? max(0, min(te - cur, kTriTake)) : 0;
       const uint64_t tb0 = __ballot(offer & 1), tb1 = __ballot(offer & 2), tb2 = __ballot(offer & 4);
       const int pre = rank_in(tb0) + 2 * rank_in(tb1) + 4 * rank_in(tb2);  // first task of this lane
-      const int ntask = min(__popcll(tb0) + 2 * __popcll(tb1) + 4 * __popcll(tb2), 64);
+      const int ntask_all = __popcll(tb0) + 2 * __popcll(tb1) + 4 * __popcll(tb2);
+      const int ntask = ntask_all < 64 ? ntask_all : 64;
       const int take = max(0, min(offer, 64 - pre));  // this lane's tasks served now
       for (int j = 0; j < take; ++j) s_task[wv][pre + j] = (static_cast<uint32_t>(cur + j) << 6) | lane;
       wave_sync();
@@ -601,7 +625,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
       const uint32_t task = tasked ? s_task[wv][lane] : static_cast<uint32_t>(lane);
       const int owner = static_cast<int>(task & 63u);
       const V3 tp = v3(lane_read_f(p.x, owner), lane_read_f(p.y, owner), lane_read_f(p.z, owner));
-      float td2 = 0.0f, tpt = 0.0f;
+      float td2 = 0.0f;
       int tid = 0;
       V3 tq = v3(0.0f, 0.0f, 0.0f);
       if (tasked) {
@@ -614,31 +638,34 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
         tq = closest_on_triangle_flat(tp, ta, tb, tc);
         const V3 e = tq - tp;
         td2 = dot(e, e);
-        tpt = walker_prune_threshold(td2, a.delta);
       }
-      // fold in record order.  Selects, no branches: pt is used only when d2
-      // is better, so d2 is not NaN; the threshold can be NaN only for a NaN
-      // point (vertices are finite), whose d2 is NaN: the select is fminf's
-      // result wherever it is used
+      // fold in record order: the lexicographic (d2, id) minimum of the best
+      // so far and the served tasks, as testing them one by one gives.  The
+      // lane reads wrap modulo 64, so lanes without task j read any lane.
       int bj = -1;
       for (int j = 0; j < kTriTake; ++j) {
         const bool has = j < take;
         if (__ballot(has) == 0) break;
-        const int src = has ? pre + j : lane;
-        const float d2 = lane_read_f(td2, src);
-        const int id = static_cast<int>(lane_read(static_cast<uint32_t>(tid), src));
-        const float ptc = lane_read_f(tpt, src);
+        const float d2 = lane_read_f(td2, pre + j);
+        const int id = static_cast<int>(lane_read(static_cast<uint32_t>(tid), pre + j));
         const bool better = has & ((d2 < bd2) | ((d2 == bd2) & (id < bid)));
-        const float pt = ptc < thr ? ptc : thr;
         bd2 = better ? d2 : bd2;
         bid = better ? id : bid;
-        thr = better ? pt : thr;
         bj = better ? j : bj;
       }
-      // the closest point of the last (so the best) improvement
-      const int qs = bj >= 0 ? pre + bj : lane;
-      const V3 q = v3(lane_read_f(tq.x, qs), lane_read_f(tq.y, qs), lane_read_f(tq.z, qs));
-      if (bj >= 0) bcp = q;
+      // the closest point of the last (so the best) improvement, and the
+      // threshold from the new best alone: one by one, each improvement
+      // would lower it in turn to the threshold of a distance >= the new
+      // best, and every such threshold is >= prune_threshold(new best) (see
+      // walker_prune_threshold), so the search stays exact either way.  A
+      // threshold can be NaN only for a NaN point (vertices are finite),
+      // whose d2 is NaN: the select is fminf's result wherever it is used
+      const V3 q = v3(lane_read_f(tq.x, pre + bj), lane_read_f(tq.y, pre + bj), lane_read_f(tq.z, pre + bj));
+      const float ptn = walker_prune_threshold(bd2, a.delta);
+      if (bj >= 0) {
+        bcp = q;
+        thr = ptn < thr ? ptn : thr;
+      }
       if (phase == kTri) {
         if (COUNT) n_tris += static_cast<uint32_t>(take);
         cur += take;

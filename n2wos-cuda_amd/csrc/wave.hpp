@@ -20,15 +20,26 @@ __device__ __forceinline__ int rank_in(uint64_t mask) {
                                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u)));
 }
 
-// v of lane `src` (0-63) of this wave, one ds_bpermute: the whole wave
-// executes it, and src names an active lane
+// v of lane src mod 64 of this wave, one ds_bpermute: the whole wave
+// executes it, and that lane is active
 __device__ __forceinline__ uint32_t lane_read(uint32_t v, int src) {
-  return static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(src << 2, static_cast<int>(v)));
+  return static_cast<uint32_t>(
+      __builtin_amdgcn_ds_bpermute(static_cast<int>((static_cast<uint32_t>(src) & 63u) << 2), static_cast<int>(v)));
 }
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
+}
+
+// This wave's global stores so far are complete (in L2) before its later
+// loads; with load_l2, which reads past the CU's vector L1, those loads see them.
+__device__ __forceinline__ void wave_global_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+__device__ __forceinline__ float load_l2(const float* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // The XCD (accelerator complex die) this wave runs on: hardware register

@@ -219,16 +219,50 @@ void check_walk_params(const n2wos_walk_params* wp) {
              "unknown source term");
 }
 
-int wos_full_grid(int source, bool ext) {
+constexpr int kSlots = 8;  // unit slots per wave
+
+// LDS ring entries per lane: 16 (default) or 8 ("ring" tuning key).
+int ring_capacity() { return tuning_value("ring", 16) == 8 ? 8 : 16; }
+
+int device_cus() {
   int dev = 0, cus = 0;
   N2_HIP(hipGetDevice(&dev));
   N2_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  constexpr int kSlots = 8;
+  return cus;
+}
+
+int wos_full_grid(int source, bool ext) {
   const bool poisson = source == N2WOS_SOURCE_POISSON_SINES;
-  const int cap = tuning_value("ring", 8) == 16 ? 16 : 8;
+  const int cap = ring_capacity();
   const int per_cu =
       poisson ? variant_blocks_per_cu<kSlots, true>(cap, ext) : variant_blocks_per_cu<kSlots, false>(cap, ext);
-  return cus * per_cu;
+  return device_cus() * per_cu;
+}
+
+// Units of a job: 64-walk units, or packed units of 64 / wpp whole points.
+int64_t job_units(int n_points, int walks_per_point) {
+  const int ps = pack_shift_for(walks_per_point);
+  if (ps >= 0) return (static_cast<int64_t>(n_points) + (64 >> ps) - 1) / (64 >> ps);
+  return static_cast<int64_t>(n_points) * ((walks_per_point + 63) / 64);
+}
+
+// Walker blocks a launch of this job can have on the current device: at most
+// one per unit and at most the largest full-device grid of any variant.
+int64_t staging_blocks(int n_points, int walks_per_point) {
+  static int per_cu_cache[64] = {};  // per device: the same for the process's lifetime
+  int dev = 0;
+  N2_HIP(hipGetDevice(&dev));
+  int per_cu = dev < 64 ? per_cu_cache[dev] : 0;
+  if (per_cu == 0) {
+    per_cu = 1;
+    for (int cap : {8, 16})
+      for (bool ext : {false, true})
+        per_cu = std::max({per_cu, variant_blocks_per_cu<kSlots, false>(cap, ext),
+                           variant_blocks_per_cu<kSlots, true>(cap, ext)});
+    if (dev < 64) per_cu_cache[dev] = per_cu;
+  }
+  const int64_t needed = (job_units(n_points, walks_per_point) + kWaves - 1) / kWaves;
+  return std::max<int64_t>(1, std::min<int64_t>(needed, static_cast<int64_t>(device_cus()) * per_cu));
 }
 
 int64_t wos_workspace_bytes(int n_points, int walks_per_point) {
@@ -236,9 +270,11 @@ int64_t wos_workspace_bytes(int n_points, int walks_per_point) {
   // counters (256 B pad) + mean + m2 (8 B each) + steps + flags (4 B each)
   // + first query per point: d2, closest xyz, tri, overflow (24 B), 256-aligned
   // + claim order: cell and order per point (8 B), cell counters (2 x 128 KiB)
+  // + unit-value staging: 256 B per unit slot of every block the launch may have
   const int64_t first = (static_cast<int64_t>(n_points) * 24 + 255) / 256 * 256;
   const int64_t order = (static_cast<int64_t>(n_points) * 8 + 255) / 256 * 256 + 2 * 4 * kOrderCells;
-  return 256 + (units * 24 + 255) / 256 * 256 + first + order;
+  const int64_t staging = staging_blocks(n_points, walks_per_point) * kWaves * kSlots * 64 * 4;
+  return 256 + (units * 24 + 255) / 256 * 256 + first + order + staging;
 }
 
 void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStream_t stream) {
@@ -291,6 +327,7 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
   int* order = reinterpret_cast<int*>(order_cell + r.n_points);
   uint32_t* order_count = reinterpret_cast<uint32_t*>(ord + (static_cast<int64_t>(r.n_points) * 8 + 255) / 256 * 256);
   uint32_t* order_cursor = order_count + kOrderCells;
+  a.unit_vals = reinterpret_cast<float*>(reinterpret_cast<char*>(order_count) + 2 * 4 * kOrderCells);
   a.order = nullptr;
   if (r.n_points >= kOrderMinPoints && tuning_value("spatial_order", 1) != 0) {
     // claim order by cell of the mesh bounds (the points themselves may lie outside)
@@ -321,22 +358,24 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
                        r.n_points);
   }
   launch_closest_point(bvh, r.d_points, r.n_points, first_d2, first_cp, first_tri, first_ovf, stream);
-  constexpr int kSlots = 8;
   const bool poisson = r.source == N2WOS_SOURCE_POISSON_SINES;
-  const int cap = tuning_value("ring", 8);
+  const int cap = ring_capacity();
   const bool ext = r.ext != 0;
   int grid = wos_full_grid(r.source, ext);
   if (r.max_blocks > 0) grid = std::min(grid, r.max_blocks);
   const int64_t needed = (static_cast<int64_t>(a.total_units) + kWaves - 1) / kWaves;
   grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(grid, needed)));
+  // the staging area holds staging_blocks() blocks, which bounds the grid
+  grid = static_cast<int>(std::min<int64_t>(grid, staging_blocks(r.n_points, r.walks_per_point)));
   a.visits = r.d_visits;
   a.refill_min = tuning_value("refill_min", 2);
   a.weight_node = tuning_value("weight_node", 10);
   // a triangle iteration tests up to 64 records of all the lanes in that
-  // phase at once, so it pays to serve it with fewer lanes waiting there
-  // (measured on the benchmark problem: 14 is 2.6% faster than 10, 5 and 7
-  // slower, 17 the same)
-  a.weight_tri = tuning_value("weight_tri", 14);
+  // phase at once, so it pays to serve it with fewer lanes waiting there,
+  // the more so the more records a leaf holds (measured on the benchmark
+  // problem: leaves of 3, 14 is 2.6% faster than 10 and 2% faster than 24;
+  // leaves of 4, 24 is 5% faster than 14 and 1.5% faster than 34)
+  a.weight_tri = tuning_value("weight_tri", bvh->leaf_size <= 3 ? 14 : 24);
   // serving walk steps sooner keeps more lanes in the traversal phases
   // (measured: weight 30 is 1.5% faster than 10 on the benchmark problem)
   a.weight_update = tuning_value("weight_update", 30);
@@ -359,8 +398,8 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
   a.path_q = r.path_q;
   a.path_len = r.path_len;
   if (r.ev_begin) N2_HIP(hipEventRecord(static_cast<hipEvent_t>(r.ev_begin), stream));
-  if (poisson) launch_variant<kSlots, true>(a, grid, stream, cap == 16 ? 16 : 8, ext);
-  else launch_variant<kSlots, false>(a, grid, stream, cap == 16 ? 16 : 8, ext);
+  if (poisson) launch_variant<kSlots, true>(a, grid, stream, cap, ext);
+  else launch_variant<kSlots, false>(a, grid, stream, cap, ext);
   if (r.ev_end) N2_HIP(hipEventRecord(static_cast<hipEvent_t>(r.ev_end), stream));
   N2_HIP(hipGetLastError());
   const int mb = 256;
@@ -419,7 +458,12 @@ extern "C" int n2wos_wos_full_grid_blocks(int source, int* blocks) {
 
 extern "C" int64_t n2wos_wos_workspace_size(int n_points, int walks_per_point) {
   if (n_points < 0 || walks_per_point < 1) return N2WOS_ERR_INVALID;
-  return wos_workspace_bytes(n_points, walks_per_point);
+  int64_t bytes = 0;
+  const int rc = guarded([&]() -> int {
+    bytes = wos_workspace_bytes(n_points, walks_per_point);
+    return N2WOS_OK;
+  });
+  return rc == N2WOS_OK ? bytes : rc;
 }
 
 namespace {
@@ -686,8 +730,8 @@ extern "C" int n2wos_nc_walks(const n2wos_bvh* bvh, const n2wos_boundary* bounda
 }
 
 extern "C" int64_t n2wos_nc_workspace_size(int n_points, int walks_per_point) {
-  if (n_points < 0 || walks_per_point < 1) return N2WOS_ERR_INVALID;
-  return wos_workspace_bytes(n_points, walks_per_point) + static_cast<int64_t>(n_points) * 28 + 256;
+  const int64_t w = n2wos_wos_workspace_size(n_points, walks_per_point);
+  return w < 0 ? w : w + static_cast<int64_t>(n_points) * 28 + 256;
 }
 
 extern "C" int n2wos_nc_combine(int n_points, int walks_per_point, int mode, const float* d_prefix,

@@ -51,6 +51,9 @@ struct WosKernelArgs {
   // holding point order[i] (null: point i), so that the walks in flight at
   // once start close together and share the mesh lines they touch
   const int* order;
+  // walk values staged per unit slot until the unit's last walk ends:
+  // [block][wave][slot][64 walks in the unit]
+  float* unit_vals;
   double* unit_mean;
   double* unit_m2;
   uint32_t* unit_steps;
@@ -138,6 +141,8 @@ struct WosRun {
   int path_len;
 };
 
+// Workspace of a walker job on the current device (its staging area for unit
+// values is sized by the device's resident walker blocks).
 int64_t wos_workspace_bytes(int n_points, int walks_per_point);
 // Validated conversions of the ABI structs (throw Error on bad values).
 Boundary to_boundary(const n2wos_boundary* boundary);

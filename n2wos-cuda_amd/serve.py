@@ -265,7 +265,7 @@ def serving_workload(eng, n_requests: int = 40, rate_per_s: float = 1.5, seed: i
     meshes = []
     for k, name in enumerate(("mesh1", "mesh2", "blob")):
         v, t = P.make_mesh(name)
-        bvh = eng.Bvh(v, t, leaf_size=3, build_method="sah")  # the walker's fastest tree
+        bvh = eng.Bvh(v, t, leaf_size=4, build_method="sah")  # the walker's fastest tree
         meshes.append((name, v, bvh, P.charges_boundary(k), WalkParams(max_steps=256, epsilon=1e-4, seed=500 + k)))
     arrivals, sizes, mesh_of = request_trace(n_requests, rate_per_s, len(meshes), seed, min_points, max_points)
     point_sets = [P.interior_points(meshes[m][2], meshes[m][1], int(n), seed=seed * 1000 + q)

@@ -63,3 +63,4 @@ struct EmuDim3 {
   unsigned x;
 };
 extern EmuDim3 threadIdx;
+extern EmuDim3 blockIdx;

@@ -18,6 +18,7 @@
 #include "hip/hip_runtime.h"
 
 EmuDim3 threadIdx;
+EmuDim3 blockIdx;  // the emulated wave is block 0
 
 namespace emu {
 enum Op { kNone = 0, kBallot, kShfl, kShflDown, kSync, kDone };
@@ -78,8 +79,8 @@ inline double __shfl_down(double v, int dlt) {
   wave_sync();                                                                                     \
   if (emu::iters > emu::iter_limit) {                                                              \
     std::fprintf(stderr, "lane %d phase %d step %d point %d walk %d cur %d te %d cnt %d "          \
-                 "scnt %d bid %d bd2 %g thr %g thr0 %g\n", lane, phase, step, point, walk, cur, te,    \
-                 cnt, scnt, bid, bd2, thr, thr0);                                                  \
+                 "scnt %d bid %d bd2 %g thr %g flags %x\n", lane, phase, step, point, walk, cur, te,  \
+                 cnt, scnt, bid, bd2, thr, wflags);                                                \
     if (lane == 0) emu::hung = 1;                                                                  \
     wave_sync();                                                                                   \
     break;                                                                                         \
@@ -113,6 +114,8 @@ inline int rank_in(uint64_t mask) {
 }
 inline void wave_sync() { emu::arrive(emu::kSync); }
 inline uint32_t lane_read(uint32_t v, int src) { return static_cast<uint32_t>(__shfl(static_cast<int>(v), src)); }
+inline void wave_global_sync() { wave_sync(); }
+inline float load_l2(const float* p) { return *p; }
 // the emulated wave's XCD: set through emu_set_unit_ranges
 inline int emu_xcd = 3;
 inline int emu_ranges = 8;
@@ -131,17 +134,27 @@ bool g_poisson = false;
 bool g_ext = false;          // run the extended walker (control variate / path record)
 n2::WosKernelArgs g_ext_args{};  // its control-variate and path fields
 
-void lane_main() {
+int g_cap = 16;  // LDS ring entries per lane: 16 (the library's default) or 8
+
+template <int CAP>
+void run_lane() {
   if (g_ext) {
     if (g_poisson)
-      n2::k_wos_persistent<8, 8, true, false, true>(g_args);
+      n2::k_wos_persistent<CAP, 8, true, false, true>(g_args);
     else
-      n2::k_wos_persistent<8, 8, false, false, true>(g_args);
+      n2::k_wos_persistent<CAP, 8, false, false, true>(g_args);
   } else if (g_poisson) {
-    n2::k_wos_persistent<8, 8, true, true>(g_args);
+    n2::k_wos_persistent<CAP, 8, true, true>(g_args);
   } else {
-    n2::k_wos_persistent<8, 8, false, true>(g_args);
+    n2::k_wos_persistent<CAP, 8, false, true>(g_args);
   }
+}
+
+void lane_main() {
+  if (g_cap == 8)
+    run_lane<8>();
+  else
+    run_lane<16>();
   emu::lanes[emu::cur].op = emu::kDone;
   swapcontext(&emu::lanes[emu::cur].ctx, &emu::sched);
 }
@@ -220,6 +233,8 @@ extern "C" int emu_walker(const void* nodes, const void* tris, float delta, cons
   a.next_unit = next_unit;
   a.unit_ranges = n2::emu_ranges;
   a.order = n2::emu_order;
+  static float unit_vals[8 * 64];  // staging of the one emulated wave's 8 unit slots
+  a.unit_vals = unit_vals;
   a.unit_mean = unit_mean;
   a.unit_m2 = unit_m2;
   a.unit_steps = unit_steps;
@@ -310,3 +325,6 @@ extern "C" void emu_path_stats(long* iters, long* lanes) {
     lanes[k] = emu::path_lanes[k];
   }
 }
+
+// LDS ring capacity of the emulated kernel for the next emu_walker calls.
+extern "C" void emu_set_cap(int cap) { g_cap = cap == 8 ? 8 : 16; }

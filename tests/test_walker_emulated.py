@@ -43,6 +43,7 @@ def emu():
     E.emu_set_order.argtypes = [vp]
     E.emu_tri_oob.restype = C.c_long
     E.emu_set_ext.argtypes = [vp, i, i, vp, vp, f, vp, vp, i]
+    E.emu_set_cap.argtypes = [i]
     return E
 
 
@@ -97,6 +98,16 @@ def _case(eng, orc, emu, mesh, npts, wpp, poisson, weights=(10, 10, 6), refill=8
 
 def test_emulated_walker_laplace(eng, orc, emu):
     _case(eng, orc, emu, "mesh1", 6, 70, False)
+
+
+def test_emulated_walker_ring8(eng, orc, emu):
+    # the 8-entry ring variant (tuning key "ring"): more spills, same walks
+    emu.emu_set_cap(8)
+    try:
+        _case(eng, orc, emu, "mesh2", 5, 64, False, weights=(10, 24, 30), refill=2, eps=1e-4, build_method="sah",
+              leaf=4)
+    finally:
+        emu.emu_set_cap(16)
 
 
 def test_emulated_walker_poisson_small_units(eng, orc, emu):
