@@ -84,7 +84,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip problem 3, time-to-RMSE and serving")
     ap.add_argument("--serve-requests", type=int, default=240)
-    ap.add_argument("--serve-rate", type=float, default=24.0, help="request arrivals per second")
+    ap.add_argument("--serve-rate", type=float, default=32.0, help="request arrivals per second")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run a 1 GiB device copy (counter calibration) and one walker solve, then exit; "
                          "used under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
