@@ -16,6 +16,7 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "api_internal.hpp"
@@ -48,6 +49,24 @@ __device__ __forceinline__ f4_t zero4() {
   return z;
 }
 
+// Entry of vertex (x, y, z) within level l: the vertex's corner in its
+// 2 x 2 x 2 brick (x & 1 | (y & 1) << 1 | (z & 1) << 2) plus 8 times the
+// brick's index, dense (bx + B by + B^2 bz, B = N/2 + 1 bricks per axis) or
+// the spatial hash of the brick coordinates mod T / 8.
+__host__ __device__ __forceinline__ uint32_t brick_entry(const GridLayout& g, int l, uint32_t x, uint32_t y,
+                                                         uint32_t z) {
+  const uint32_t bx = x >> 1, by = y >> 1, bz = z >> 1;
+  const uint32_t corner = (x & 1u) | ((y & 1u) << 1) | ((z & 1u) << 2);
+  uint32_t brick;
+  if (g.dense[l]) {
+    const uint32_t nb = static_cast<uint32_t>(g.res[l] >> 1) + 1u;
+    brick = bx + by * nb + bz * nb * nb;
+  } else {
+    brick = (bx ^ (by * 2654435761u) ^ (bz * 805459861u)) & ((g.table_size >> 3) - 1u);
+  }
+  return brick * 8u + corner;
+}
+
 // Cell corner indices and trilinear weights of point u (unit cube) on level l.
 __host__ __device__ __forceinline__ void level_corners(const GridLayout& g, int l, float ux, float uy, float uz,
                                                        uint32_t idx[8], float w[8]) {
@@ -58,17 +77,11 @@ __host__ __device__ __forceinline__ void level_corners(const GridLayout& g, int 
   cy = cy < 0 ? 0 : (cy > n - 1 ? n - 1 : cy);
   cz = cz < 0 ? 0 : (cz > n - 1 ? n - 1 : cz);
   const float fx = sx - static_cast<float>(cx), fy = sy - static_cast<float>(cy), fz = sz - static_cast<float>(cz);
-  const uint32_t stride = static_cast<uint32_t>(n + 1);
   for (int c = 0; c < 8; ++c) {
     const uint32_t x = static_cast<uint32_t>(cx + (c & 1));
     const uint32_t y = static_cast<uint32_t>(cy + ((c >> 1) & 1));
     const uint32_t z = static_cast<uint32_t>(cz + ((c >> 2) & 1));
-    uint32_t local;
-    if (g.dense[l])
-      local = x + y * stride + z * stride * stride;
-    else
-      local = (x ^ (y * 2654435761u) ^ (z * 805459861u)) & (g.table_size - 1u);
-    idx[c] = g.offset[l] + local;
+    idx[c] = g.offset[l] + brick_entry(g, l, x, y, z);
     w[c] = ((c & 1) ? fx : 1.0f - fx) * (((c >> 1) & 1) ? fy : 1.0f - fy) * (((c >> 2) & 1) ? fz : 1.0f - fz);
   }
 }
@@ -127,56 +140,93 @@ __device__ __forceinline__ h4_t frag_trans(const _Float16* m, int cols, int r, i
   return a;
 }
 
-// Value of v in lane J of this lane's quad (four consecutive lanes), by DPP
-// quad_perm [J, J, J, J].  Callers run it with the quad's four lanes active.
-template <int J>
-__device__ __forceinline__ float quad_lane(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), J * 0x55, 0xF, 0xF, false));
+// Value of v in lane S of this lane's row of 16 lanes (DPP row_newbcast).
+template <int S>
+__device__ __forceinline__ float row_lane(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x150 + S, 0xF, 0xF, false));
 }
 
-// Grid-gradient scatter of one level for the four samples of a quad: for
-// sample j (unit-cube position jx/jy/jz[j], feature gradients j0/j1[j]) lane
-// k of the quad adds w_c * g_{k & 1} for corner c = 2 cp + (k >> 1), cp =
-// 0..3.  Lanes k = 0..3 thus cover [f0 f1] of corner c and of its +x
-// neighbour, 16 contiguous bytes on dense levels (and on hashed levels for
-// even x, where x + 1 only flips bit 0 of the hash), so each atomic
-// instruction sends a quad's four updates as one memory request: the atomic
-// rate is set by requests, not by dwords (measured on MI355X: 21 G requests/s
-// whether a request carries one dword or four).  The corner weights and
-// indices are level_corners' own formulas.
-__device__ __forceinline__ void scatter_level(const GridLayout& gl, int l, int qk, const float jx[4],
-                                              const float jy[4], const float jz[4], const float jlive[4],
-                                              const float j0[4], const float j1[4], float* __restrict__ grid_grad) {
-  const int n = gl.res[l];
-  const float nf = static_cast<float>(n);
-  const uint32_t stride = static_cast<uint32_t>(n + 1);
-  const bool dense = gl.dense[l] != 0;
-  const uint32_t base = gl.offset[l], tmask = gl.table_size - 1u;
-  const int bx = qk >> 1;
+// The four levels one row of the wave scatters (lane (i, g) holds the input
+// gradients of levels 2g, 2g+1, 8+2g, 9+2g of sample i): level, resolution,
+// dense flag, first grid float (2 x first entry) and bricks per axis.
+struct RowLevels {
+  int l[4];
+  int n[4];
+  int dense[4];
+  uint32_t base2[4];
+  uint32_t nb[4];
+};
+
+__device__ __forceinline__ RowLevels row_levels(const GridLayout& gl, int g) {
+  RowLevels r;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const float g0 = j0[j], g1 = j1[j];
-    if (jlive[j] == 0.0f || (g0 == 0.0f && g1 == 0.0f)) continue;
-    const float sx = jx[j] * nf, sy = jy[j] * nf, sz = jz[j] * nf;
+  for (int k = 0; k < 4; ++k) {
+    const int l = 8 * (k >> 1) + 2 * g + (k & 1);
+    const bool on = l < gl.n_levels;
+    r.l[k] = on ? l : -1;
+    r.n[k] = on ? gl.res[l] : 1;
+    r.dense[k] = on ? gl.dense[l] : 0;
+    r.base2[k] = on ? 2u * gl.offset[l] : 0u;
+    r.nb[k] = static_cast<uint32_t>(r.n[k] >> 1) + 1u;
+  }
+  return r;
+}
+
+// Grid-gradient scatter of sample S of the tile at one level per row: the
+// row adds w_c g_f of the sample's cell into the 2 x 2 x 2 bricks the cell
+// touches (1, 2, 4 or 8: two along an axis where the cell starts at an odd
+// vertex).  Lane r of the row stands for dword r of a brick's 64 bytes
+// (corner r >> 1, feature r & 1).  Along each axis exactly one of the cell's
+// bricks holds a cell vertex at this lane's corner: for an even first vertex
+// c the brick c/2 (vertex c + o for corner offset o), for an odd one brick
+// (c - 1)/2 + 1 - o (vertex c + 1 - o).  So every lane adds exactly one
+// value, one of the cell's 8 x 2, and the row's one atomic instruction sends
+// one 64-byte request per brick it touches: the memory-side atomic rate is
+// set by requests, whatever their size up to 64 bytes (21 G requests/s on
+// MI355X for 1 to 16 dwords).  The corner weights are level_corners' own
+// products.
+template <int S>
+__device__ __forceinline__ void scatter_sample(const GridLayout& gl, const RowLevels& rl, float ux, float uy, float uz,
+                                               float live, const f4_t (&dx)[2], int r,
+                                               float* __restrict__ grid_grad) {
+  const float px = row_lane<S>(ux), py = row_lane<S>(uy), pz = row_lane<S>(uz);
+  const float pl = row_lane<S>(live);
+  const uint32_t rc = static_cast<uint32_t>(r >> 1);
+  const uint32_t ox = rc & 1u, oy = (rc >> 1) & 1u, oz = rc >> 2;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float g0 = row_lane<S>(dx[k >> 1][2 * (k & 1)]);
+    const float g1 = row_lane<S>(dx[k >> 1][2 * (k & 1) + 1]);
+    if (rl.l[k] < 0 || pl == 0.0f || (g0 == 0.0f && g1 == 0.0f)) continue;
+    const int n = rl.n[k];
+    const float nf = static_cast<float>(n);
+    const float sx = px * nf, sy = py * nf, sz = pz * nf;
     int cx = static_cast<int>(floorf(sx)), cy = static_cast<int>(floorf(sy)), cz = static_cast<int>(floorf(sz));
     cx = cx < 0 ? 0 : (cx > n - 1 ? n - 1 : cx);
     cy = cy < 0 ? 0 : (cy > n - 1 ? n - 1 : cy);
     cz = cz < 0 ? 0 : (cz > n - 1 ? n - 1 : cz);
     const float fx = sx - static_cast<float>(cx), fy = sy - static_cast<float>(cy), fz = sz - static_cast<float>(cz);
-    const uint32_t x = static_cast<uint32_t>(cx + bx);
-    const float wx = bx ? fx : 1.0f - fx;
-    const float gk = (qk & 1) ? g1 : g0;
-#pragma unroll
-    for (int cp = 0; cp < 4; ++cp) {
-      const uint32_t y = static_cast<uint32_t>(cy + (cp & 1));
-      const uint32_t z = static_cast<uint32_t>(cz + (cp >> 1));
-      const float wy = (cp & 1) ? fy : 1.0f - fy;
-      const float wz = (cp >> 1) ? fz : 1.0f - fz;
-      const uint32_t local = dense ? x + y * stride + z * stride * stride
-                                   : (x ^ (y * 2654435761u) ^ (z * 805459861u)) & tmask;
-      atomicAdd(grid_grad + 2 * static_cast<size_t>(base + local) + (qk & 1), wx * wy * wz * gk);
-    }
+    // this lane's cell vertex along each axis: offset d (0 or 1) from the
+    // first vertex, and the brick that holds it
+    const uint32_t ux0 = static_cast<uint32_t>(cx), uy0 = static_cast<uint32_t>(cy), uz0 = static_cast<uint32_t>(cz);
+    const uint32_t dxo = (ux0 & 1u) ? 1u - ox : ox, dyo = (uy0 & 1u) ? 1u - oy : oy, dzo = (uz0 & 1u) ? 1u - oz : oz;
+    const uint32_t bx = (ux0 + dxo) >> 1, by = (uy0 + dyo) >> 1, bz = (uz0 + dzo) >> 1;
+    const float wx = dxo ? fx : 1.0f - fx;
+    const float wy = dyo ? fy : 1.0f - fy;
+    const float wz = dzo ? fz : 1.0f - fz;
+    const float gf = (r & 1) ? g1 : g0;
+    const uint32_t brick = rl.dense[k] ? bx + by * rl.nb[k] + bz * rl.nb[k] * rl.nb[k]
+                                       : (bx ^ (by * 2654435761u) ^ (bz * 805459861u)) & ((gl.table_size >> 3) - 1u);
+    atomicAdd(grid_grad + rl.base2[k] + 16u * static_cast<size_t>(brick) + static_cast<uint32_t>(r),
+              wx * wy * wz * gf);
   }
+}
+
+template <int... S>
+__device__ __forceinline__ void scatter_tile(const GridLayout& gl, const RowLevels& rl, float ux, float uy, float uz,
+                                             float live, const f4_t (&dx)[2], int r, float* __restrict__ grid_grad,
+                                             std::integer_sequence<int, S...>) {
+  (scatter_sample<S>(gl, rl, ux, uy, uz, live, dx, r, grid_grad), ...);
 }
 
 // Forward of one 16-sample tile through the MLP with L hidden layers of width
@@ -288,7 +338,7 @@ __global__ void __launch_bounds__(kCacheBlock) k_cache_train_pass1(GridLayout gl
                                                                   float offset, float grad_scale, TrainStore st,
                                                                   float* __restrict__ grid_grad_copies,
                                                                   float* __restrict__ loss_sum, uint32_t copy_mask,
-                                                                  size_t gstride) {
+                                                                  size_t gstride, int* __restrict__ bad) {
   constexpr int T = H / 16;
   float* __restrict__ grid_grad = grid_grad_copies + (xcd_id() & copy_mask) * gstride;
   __shared__ _Float16 w[cache_weights(H, L)];
@@ -299,7 +349,9 @@ __global__ void __launch_bounds__(kCacheBlock) k_cache_train_pass1(GridLayout gl
   const int wave = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / 64);
   const int tiles = st.batch / 16;
   const size_t HB = static_cast<size_t>(H) * st.batch;
+  const RowLevels rl = row_levels(gl, g);
   float loss = 0.0f;
+  bool finite = true;  // every dX value this lane scattered (and so every grid-gradient term)
   for (int tile = wave; tile < tiles; tile += waves) {
     const int s = tile * 16 + i;
     const bool live = s < n;
@@ -353,32 +405,22 @@ __global__ void __launch_bounds__(kCacheBlock) k_cache_train_pass1(GridLayout gl
     }
     // input-feature gradients dX = A1^T dZ1 (32 x 16), computed by the whole
     // wave (an MFMA reads every lane's operands), then scattered to the grid
-    // a quad at a time (scatter_level): each quad's samples, positions and
-    // gradients are shared among its lanes first, with every lane active
-    const int qk = lane & 3;
-    const float jx[4] = {quad_lane<0>(ux), quad_lane<1>(ux), quad_lane<2>(ux), quad_lane<3>(ux)};
-    const float jy[4] = {quad_lane<0>(uy), quad_lane<1>(uy), quad_lane<2>(uy), quad_lane<3>(uy)};
-    const float jz[4] = {quad_lane<0>(uz), quad_lane<1>(uz), quad_lane<2>(uz), quad_lane<3>(uz)};
-    const float lv = live ? 1.0f : 0.0f;
-    const float jlive[4] = {quad_lane<0>(lv), quad_lane<1>(lv), quad_lane<2>(lv), quad_lane<3>(lv)};
+    // brick by brick, one sample and one level per row at a time
+    f4_t dx[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       f4_t acc = zero4();
 #pragma unroll
       for (int k = 0; k < T; ++k) acc = mfma16(frag_trans(w, kCacheIn, 16 * t + i, 16 * k + 4 * g), dz[k], acc);
-#pragma unroll
-      for (int hh = 0; hh < 2; ++hh) {
-        const float a0 = acc[2 * hh], a1 = acc[2 * hh + 1];
-        const float j0[4] = {quad_lane<0>(a0), quad_lane<1>(a0), quad_lane<2>(a0), quad_lane<3>(a0)};
-        const float j1[4] = {quad_lane<0>(a1), quad_lane<1>(a1), quad_lane<2>(a1), quad_lane<3>(a1)};
-        const int l = 8 * t + 2 * g + hh;  // the same in the four lanes of a quad
-        if (l < gl.n_levels) scatter_level(gl, l, qk, jx, jy, jz, jlive, j0, j1, grid_grad);
-      }
+      dx[t] = acc;
+      finite = finite && isfinite(acc[0]) && isfinite(acc[1]) && isfinite(acc[2]) && isfinite(acc[3]);
     }
+    scatter_tile(gl, rl, ux, uy, uz, live ? 1.0f : 0.0f, dx, i, grid_grad, std::make_integer_sequence<int, 16>{});
   }
   // loss: sum over the wave's lanes (only g == 0 lanes carry values)
   for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
   if (lane == 0) atomicAdd(loss_sum, loss);
+  if (!finite) *bad = 1;
 }
 
 // Training pass 2: weight gradients dA = dZ H_prev^T over the batch (K =
@@ -389,7 +431,8 @@ __global__ void __launch_bounds__(kCacheBlock) k_cache_train_pass1(GridLayout gl
 // tile are zero).
 template <int ROWS_T, int COLS_T>
 __device__ __forceinline__ void wgrad_layer(const _Float16* __restrict__ dz, const _Float16* __restrict__ hp,
-                                            size_t B, int k0, int k1, int i, int g, int cols, float* __restrict__ wg) {
+                                            size_t B, int k0, int k1, int i, int g, int cols, float* __restrict__ wg,
+                                            int* __restrict__ bad) {
   constexpr int R = ROWS_T == 0 ? 1 : ROWS_T;
   f4_t acc[R * COLS_T];
 #pragma unroll
@@ -424,12 +467,14 @@ __device__ __forceinline__ void wgrad_layer(const _Float16* __restrict__ dz, con
       for (int e = 0; e < 4; ++e) {
         const int row = 16 * r + 4 * g + e;
         if (ROWS_T == 0 && row != 0) continue;
+        if (!isfinite(acc[r * COLS_T + c][e])) *bad = 1;
         atomicAdd(wg + row * cols + 16 * c + i, acc[r * COLS_T + c][e]);
       }
 }
 
 template <int H, int L>
-__global__ void __launch_bounds__(kCacheBlock) k_cache_wgrad(TrainStore st, int chunk, float* __restrict__ w_grad) {
+__global__ void __launch_bounds__(kCacheBlock) k_cache_wgrad(TrainStore st, int chunk, float* __restrict__ w_grad,
+                                                             int* __restrict__ bad) {
   constexpr int T = H / 16;
   const int layer = static_cast<int>(blockIdx.y);
   const int lane = static_cast<int>(threadIdx.x & 63);
@@ -441,44 +486,66 @@ __global__ void __launch_bounds__(kCacheBlock) k_cache_wgrad(TrainStore st, int 
   const size_t B = static_cast<size_t>(st.batch);
   const size_t HB = static_cast<size_t>(H) * B;
   if (layer == 0) {
-    wgrad_layer<T, 2>(st.dz, st.x, B, k0, k1, i, g, kCacheIn, w_grad);
+    wgrad_layer<T, 2>(st.dz, st.x, B, k0, k1, i, g, kCacheIn, w_grad, bad);
   } else if (layer < L) {
     wgrad_layer<T, T>(st.dz + layer * HB, st.h + (layer - 1) * HB, B, k0, k1, i, g, H,
-                      w_grad + cache_off_hidden(H, layer));
+                      w_grad + cache_off_hidden(H, layer), bad);
   } else {
-    wgrad_layer<0, T>(st.dy, st.h + (L - 1) * HB, B, k0, k1, i, g, H, w_grad + cache_off_out(H, L));
+    wgrad_layer<0, T>(st.dy, st.h + (L - 1) * HB, B, k0, k1, i, g, H, w_grad + cache_off_out(H, L), bad);
   }
 }
 
-// Adam (Kingma & Ba) on FP32 parameters with unscaled gradients, the sum of
-// `copies` gradient copies n floats apart (in copy order); zeroes them.
-// Skips the update (and still zeroes) when *bad is set.
-__global__ void k_adam(float* __restrict__ p, float* __restrict__ grad, float* __restrict__ m, float* __restrict__ v,
-                       size_t n, int copies, float lr_t, float beta1, float beta2, float eps, float inv_scale,
-                       const int* __restrict__ bad) {
-  const bool skip = *bad != 0;
-  for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; k < n;
-       k += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    float gs = grad[k];
-    grad[k] = 0.0f;
-    for (int c = 1; c < copies; ++c) {
-      gs += grad[c * n + k];
-      grad[c * n + k] = 0.0f;
+// One Adam step (Kingma & Ba) on FP32 parameters with unscaled gradients:
+// the weights in blocks [0, wblocks), which also write their FP16 copy for
+// the MFMA kernels, and the grid in the other blocks, whose gradient is the
+// sum of its copies in copy order; the gradients are zeroed as they are read.
+// Skipped (the gradients still zeroed) when *bad is set.
+struct AdamArgs {
+  float *w, *w_grad, *w_m, *w_v;
+  _Float16* wh;
+  int n_w;
+  float *grid, *grid_grad, *grid_m, *grid_v;
+  size_t n_grid;
+  int copies;
+  float lr_t, beta1, beta2, eps, inv_scale;
+  const int* bad;
+  int wblocks;
+};
+
+__device__ __forceinline__ float adam_update(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+                                             size_t k, float gk, const AdamArgs& a) {
+  const float mk = a.beta1 * m[k] + (1.0f - a.beta1) * gk;
+  const float vk = a.beta2 * v[k] + (1.0f - a.beta2) * gk * gk;
+  m[k] = mk;
+  v[k] = vk;
+  const float pk = p[k] - a.lr_t * mk / (sqrtf(vk) + a.eps);
+  p[k] = pk;
+  return pk;
+}
+
+__global__ void k_adam_step(AdamArgs a) {
+  const bool skip = *a.bad != 0;
+  if (static_cast<int>(blockIdx.x) < a.wblocks) {
+    for (int k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); k < a.n_w; k += a.wblocks * blockDim.x) {
+      const float gk = a.w_grad[k] * a.inv_scale;
+      a.w_grad[k] = 0.0f;
+      if (skip) continue;
+      a.wh[k] = static_cast<_Float16>(adam_update(a.w, a.w_m, a.w_v, static_cast<size_t>(k), gk, a));
     }
-    const float gk = gs * inv_scale;
-    if (skip) continue;
-    const float mk = beta1 * m[k] + (1.0f - beta1) * gk;
-    const float vk = beta2 * v[k] + (1.0f - beta2) * gk * gk;
-    m[k] = mk;
-    v[k] = vk;
-    p[k] = p[k] - lr_t * mk / (sqrtf(vk) + eps);
+    return;
   }
-}
-
-__global__ void k_check_finite(const float* __restrict__ a, size_t n, int* __restrict__ bad) {
-  for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; k < n;
-       k += static_cast<size_t>(gridDim.x) * blockDim.x)
-    if (!isfinite(a[k])) *bad = 1;
+  const size_t stride = static_cast<size_t>(gridDim.x - a.wblocks) * blockDim.x;
+  for (size_t k = (blockIdx.x - a.wblocks) * static_cast<size_t>(blockDim.x) + threadIdx.x; k < a.n_grid;
+       k += stride) {
+    float gs = a.grid_grad[k];
+    a.grid_grad[k] = 0.0f;
+    for (int c = 1; c < a.copies; ++c) {
+      gs += a.grid_grad[c * a.n_grid + k];
+      a.grid_grad[c * a.n_grid + k] = 0.0f;
+    }
+    if (skip) continue;
+    adam_update(a.grid, a.grid_m, a.grid_v, k, gs * a.inv_scale, a);
+  }
 }
 
 __global__ void k_to_half(const float* __restrict__ a, _Float16* __restrict__ h, int n) {
@@ -486,7 +553,10 @@ __global__ void k_to_half(const float* __restrict__ a, _Float16* __restrict__ h,
   if (k < n) h[k] = static_cast<_Float16>(a[k]);
 }
 
-__global__ void k_zero_int(int* p) { *p = 0; }
+__global__ void k_zero_two(int* p, int* q) {
+  *p = 0;
+  *q = 0;
+}
 
 __global__ void k_init_uniform(float* __restrict__ p, size_t n, float lo, float hi, uint32_t seed, uint32_t tag) {
   for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; k < n;
@@ -577,7 +647,7 @@ void launch_pass1(const n2wos_cache* c, const float* d_points, const float* d_ta
     hipLaunchKernelGGL((k_cache_train_pass1<decltype(HC)::value, decltype(LC)::value>), dim3(blocks),
                        dim3(kCacheBlock), 0, s, c->layout, c->grid, c->wh, d_points, d_targets, n, c->out_offset,
                        grad_scale, st, c->grid_grad, c->loss, static_cast<uint32_t>(c->grad_copies - 1),
-                       grid_floats(c));
+                       grid_floats(c), c->bad);
   });
 }
 
@@ -586,7 +656,8 @@ void launch_wgrad(const n2wos_cache* c, const TrainStore& st, hipStream_t s) {
   const int waves = (st.batch + chunk - 1) / chunk;
   with_shape(c, [&](auto HC, auto LC) {
     hipLaunchKernelGGL((k_cache_wgrad<decltype(HC)::value, decltype(LC)::value>),
-                       dim3((waves + 3) / 4, decltype(LC)::value + 1), dim3(kCacheBlock), 0, s, st, chunk, c->w_grad);
+                       dim3((waves + 3) / 4, decltype(LC)::value + 1), dim3(kCacheBlock), 0, s, st, chunk, c->w_grad,
+                       c->bad);
   });
 }
 
@@ -626,9 +697,11 @@ extern "C" int n2wos_cache_create(const n2wos_cache_config* cfg, n2wos_cache** o
       for (int l = 0; l < kCacheLevels; ++l) {
         const double nres = std::floor(static_cast<double>(cfg->base_resolution) * std::pow(cfg->growth, l));
         L.res[l] = static_cast<int>(nres);
-        const double verts = std::pow(nres + 1.0, 3.0);
-        L.dense[l] = verts <= static_cast<double>(L.table_size) ? 1 : 0;
-        L.size[l] = L.dense[l] ? static_cast<uint32_t>((static_cast<uint64_t>(verts) + 7) / 8 * 8) : L.table_size;
+        // 8 entries per brick of 2 x 2 x 2 vertices, N/2 + 1 bricks per axis
+        const double nb = std::floor(nres / 2.0) + 1.0;
+        const double entries = 8.0 * nb * nb * nb;
+        L.dense[l] = entries <= static_cast<double>(L.table_size) ? 1 : 0;
+        L.size[l] = L.dense[l] ? static_cast<uint32_t>(entries) : L.table_size;
         L.offset[l] = static_cast<uint32_t>(off);
         if (l < L.n_levels) off += L.size[l];
       }
@@ -737,24 +810,27 @@ extern "C" int n2wos_cache_train_step(n2wos_cache* c, const float* d_points, con
                "batch of " + std::to_string(n) + " exceeds the cache's max_batch (" +
                    std::to_string(c->store_batch) + ")");
     const TrainStore st = make_store(c, batch);
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, c->bad);
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, reinterpret_cast<int*>(c->loss));
+    // the step is skipped when a gradient term is not finite: pass 1 and the
+    // weight-gradient kernel flag it where they compute the terms (a grid
+    // gradient is a sum of the scattered terms, far too few and too small to
+    // overflow FP32 when each is finite)
+    hipLaunchKernelGGL(k_zero_two, dim3(1), dim3(1), 0, s, c->bad, reinterpret_cast<int*>(c->loss));
     launch_pass1(c, d_points, d_targets, n, c->loss_scale / static_cast<float>(n), st, s);
     launch_wgrad(c, st, s);
-    hipLaunchKernelGGL(k_check_finite, dim3(64), dim3(256), 0, s, c->w_grad, static_cast<size_t>(c->n_w), c->bad);
-    hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, s, c->grid_grad,
-                       grid_floats(c) * static_cast<size_t>(c->grad_copies), c->bad);
     c->step += 1;
     const double bc1 = 1.0 - std::pow(static_cast<double>(c->beta1), static_cast<double>(c->step));
     const double bc2 = 1.0 - std::pow(static_cast<double>(c->beta2), static_cast<double>(c->step));
-    const float lr_t = static_cast<float>(c->lr * std::sqrt(bc2) / bc1);
-    const float eps_t = static_cast<float>(c->eps * std::sqrt(bc2));
-    const float inv = 1.0f / c->loss_scale;
-    hipLaunchKernelGGL(k_adam, dim3(64), dim3(256), 0, s, c->w, c->w_grad, c->w_m, c->w_v,
-                       static_cast<size_t>(c->n_w), 1, lr_t, c->beta1, c->beta2, eps_t, inv, c->bad);
-    hipLaunchKernelGGL(k_adam, dim3(2048), dim3(256), 0, s, c->grid, c->grid_grad, c->grid_m, c->grid_v,
-                       grid_floats(c), c->grad_copies, lr_t, c->beta1, c->beta2, eps_t, inv, c->bad);
-    hipLaunchKernelGGL(k_to_half, dim3((c->n_w + 255) / 256), dim3(256), 0, s, c->w, c->wh, c->n_w);
+    AdamArgs a;
+    a.w = c->w; a.w_grad = c->w_grad; a.w_m = c->w_m; a.w_v = c->w_v; a.wh = c->wh; a.n_w = c->n_w;
+    a.grid = c->grid; a.grid_grad = c->grid_grad; a.grid_m = c->grid_m; a.grid_v = c->grid_v;
+    a.n_grid = grid_floats(c); a.copies = c->grad_copies;
+    a.lr_t = static_cast<float>(c->lr * std::sqrt(bc2) / bc1);
+    a.beta1 = c->beta1; a.beta2 = c->beta2;
+    a.eps = static_cast<float>(c->eps * std::sqrt(bc2));
+    a.inv_scale = 1.0f / c->loss_scale;
+    a.bad = c->bad;
+    a.wblocks = (c->n_w + 255) / 256;
+    hipLaunchKernelGGL(k_adam_step, dim3(a.wblocks + 2048), dim3(256), 0, s, a);
     N2_HIP(hipGetLastError());
     if (d_loss_out) N2_HIP(hipMemcpyAsync(d_loss_out, c->loss, 4, hipMemcpyDeviceToDevice, s));
     return N2WOS_OK;
@@ -857,7 +933,7 @@ extern "C" int n2wos_cache_gradients(n2wos_cache* c, const float* d_points, cons
     const size_t gf = grid_floats(c);
     std::vector<float> tmp;
     if (grid_grad) {
-      // the sum of the per-XCD copies, in copy order (as k_adam sums them)
+      // the sum of the per-XCD copies, in copy order (as k_adam_step sums them)
       tmp.resize(gf * static_cast<size_t>(c->grad_copies));
       N2_HIP(hipMemcpy(tmp.data(), c->grid_grad, tmp.size() * 4, hipMemcpyDeviceToHost));
       for (size_t k = 0; k < gf; ++k) {

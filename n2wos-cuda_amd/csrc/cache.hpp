@@ -3,10 +3,14 @@
 // Encoding (Mueller, Evans, Schied, Keller 2022, "Instant Neural Graphics
 // Primitives with a Multiresolution Hash Encoding"): L levels of F = 2
 // features; level l has N_l = floor(N_min * b^l) cells per axis over the unit
-// cube, i.e. (N_l + 1)^3 vertices; levels whose vertex count fits in T entries
-// are indexed densely (x + y (N_l+1) + z (N_l+1)^2), the rest by the spatial
-// hash (x * 1 ^ y * 2654435761 ^ z * 805459861) mod T.  Features are
-// trilinearly interpolated from the 8 cell corners.  Tables are FP32.
+// cube, i.e. (N_l + 1)^3 vertices.  The vertices are grouped in bricks of
+// 2 x 2 x 2 (brick (x/2, y/2, z/2), 8 entries = 64 bytes, one memory request
+// for a gradient scatter and at most one cache line per brick a lookup
+// touches): vertex (x, y, z) is entry 8 brick + (x & 1 | (y & 1) << 1 |
+// (z & 1) << 2).  Levels whose B^3 bricks (B = N_l/2 + 1) fit in T entries
+// index bricks densely (bx + B by + B^2 bz), the rest by the spatial hash of
+// the brick, (bx * 1 ^ by * 2654435761 ^ bz * 805459861) mod T/8.  Features
+// are trilinearly interpolated from the 8 cell corners.  Tables are FP32.
 //
 // MLP: 2L = 32 inputs -> `layers` hidden layers of width H (ReLU) -> 1 output,
 // no biases (the layout of a fully fused MLP); H in {16, 32, 64}, layers in

@@ -1260,8 +1260,11 @@ int orc_bvh4_stack_profile(const void* nodes, const void* tris, float delta, con
 
 /* ------------------------------------------------------------------ */
 /* Learned-cache oracle in FP64: multiresolution hash-grid encoding (Mueller et
- * al. 2022: N_l = floor(N_min b^l), dense levels when (N_l + 1)^3 <= T,
- * spatial hash x*1 ^ y*2654435761 ^ z*805459861 mod T, trilinear weights) and
+ * al. 2022: N_l = floor(N_min b^l), trilinear weights) with the vertices
+ * grouped in 2 x 2 x 2 bricks of 8 entries: vertex (x, y, z) is entry
+ * 8 brick + (x & 1 | (y & 1) << 1 | (z & 1) << 2) of brick (x/2, y/2, z/2),
+ * indexed densely (B = N_l/2 + 1 bricks per axis) when 8 B^3 <= T, else by
+ * the spatial hash bx*1 ^ by*2654435761 ^ bz*805459861 mod T/8, and
  * a bias-free ReLU MLP 2L -> H -> ... -> H -> 1 (`layers` hidden layers of
  * width H <= 64), with the gradients of the mean squared error by reverse
  * accumulation.  Weight buffer layout (orc_w_off): A1 [H x 2L] (32 columns
@@ -1292,15 +1295,30 @@ int orc_cache_layout(int n_levels, int log2_t, int base_res, double growth, uint
   uint64_t off = 0;
   for (int l = 0; l < 16; ++l) {
     const double n = floor((double)base_res * pow(growth, (double)l));
-    const double verts = (n + 1.0) * (n + 1.0) * (n + 1.0);
+    const double nb = floor(n / 2.0) + 1.0;
+    const double entries = 8.0 * nb * nb * nb;
     res[l] = (int)n;
-    dense[l] = verts <= (double)T;
-    sizes[l] = dense[l] ? (uint32_t)(((uint64_t)verts + 7) / 8 * 8) : T;
+    dense[l] = entries <= (double)T;
+    sizes[l] = dense[l] ? (uint32_t)entries : T;
     offsets[l] = (uint32_t)off;
     if (l < n_levels) off += sizes[l];
   }
   *total = (uint32_t)off;
   return 0;
+}
+
+/* Entry of vertex (x, y, z) within level l (brick layout above). */
+static uint32_t orc_entry(const orc_grid* g, int l, uint32_t x, uint32_t y, uint32_t z) {
+  const uint32_t bx = x >> 1, by = y >> 1, bz = z >> 1;
+  const uint32_t corner = (x & 1u) | ((y & 1u) << 1) | ((z & 1u) << 2);
+  uint32_t brick;
+  if (g->dense[l]) {
+    const uint32_t nb = (uint32_t)(g->res[l] / 2) + 1u;
+    brick = bx + by * nb + bz * nb * nb;
+  } else {
+    brick = (bx ^ (by * 2654435761u) ^ (bz * 805459861u)) & (g->table_size / 8u - 1u);
+  }
+  return 8u * brick + corner;
 }
 
 static void orc_corners(const orc_grid* g, int l, const double u[3], uint32_t idx[8], double w[8]) {
@@ -1314,13 +1332,10 @@ static void orc_corners(const orc_grid* g, int l, const double u[3], uint32_t id
     c[a] = ci;
     f[a] = s - (double)ci;
   }
-  const uint32_t stride = (uint32_t)(n + 1);
   for (int k = 0; k < 8; ++k) {
     const uint32_t x = (uint32_t)(c[0] + (k & 1)), y = (uint32_t)(c[1] + ((k >> 1) & 1)),
                    z = (uint32_t)(c[2] + ((k >> 2) & 1));
-    const uint32_t local = g->dense[l] ? x + y * stride + z * stride * stride
-                                       : (x ^ (y * 2654435761u) ^ (z * 805459861u)) & (g->table_size - 1u);
-    idx[k] = g->offset[l] + local;
+    idx[k] = g->offset[l] + orc_entry(g, l, x, y, z);
     w[k] = ((k & 1) ? f[0] : 1.0 - f[0]) * (((k >> 1) & 1) ? f[1] : 1.0 - f[1]) * (((k >> 2) & 1) ? f[2] : 1.0 - f[2]);
   }
 }
@@ -1353,13 +1368,10 @@ static void orc_corners_f32(const orc_grid* g, int l, const float u[3], uint32_t
     c[a] = ci;
     f[a] = sa - (float)ci;
   }
-  const uint32_t stride = (uint32_t)(n + 1);
   for (int k = 0; k < 8; ++k) {
     const uint32_t x = (uint32_t)(c[0] + (k & 1)), y = (uint32_t)(c[1] + ((k >> 1) & 1)),
                    z = (uint32_t)(c[2] + ((k >> 2) & 1));
-    const uint32_t local = g->dense[l] ? x + y * stride + z * stride * stride
-                                       : (x ^ (y * 2654435761u) ^ (z * 805459861u)) & (g->table_size - 1u);
-    idx[k] = g->offset[l] + local;
+    idx[k] = g->offset[l] + orc_entry(g, l, x, y, z);
     const float wx = (k & 1) ? f[0] : 1.0f - f[0];
     const float wy = ((k >> 1) & 1) ? f[1] : 1.0f - f[1];
     const float wz = ((k >> 2) & 1) ? f[2] : 1.0f - f[2];

@@ -217,3 +217,28 @@ def test_mlp_shapes_match_fp16_oracle(eng, orc, torch_gpu, h, layers):
         c.train_step(p_t, t_t)
     _, _, loss2 = c.gradients(p_t, t_t)
     assert loss2 < loss
+
+
+@pytest.mark.parametrize("bad_value", [float("nan"), float("inf")])
+def test_non_finite_step_is_skipped(eng, torch_gpu, bad_value):
+    """A batch with one non-finite target makes that step's gradients
+    non-finite: the step leaves every parameter as it was (the kernels flag
+    the terms where they compute them), and the next finite batch trains."""
+    from n2wos_amd import cache as K
+    torch = torch_gpu
+    c = K.Cache(K.CacheConfig(n_levels=4, log2_table_size=12, base_resolution=4, growth=1.5, seed=5, max_batch=4096))
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x = (torch.rand((4096, 3), device="cuda", generator=g) * 2 - 1).contiguous()
+    y = (x[:, 0] * x[:, 1]).contiguous()
+    c.train_step(x, y)
+    grid0, w0 = c.get_params()
+    yb = y.clone()
+    yb[1234] = bad_value
+    c.train_step(x, yb.contiguous())
+    grid1, w1 = c.get_params()
+    assert np.array_equal(grid0.view(np.uint32), grid1.view(np.uint32))
+    assert np.array_equal(w0.view(np.uint32), w1.view(np.uint32))
+    c.train_step(x, y)
+    grid2, w2 = c.get_params()
+    assert np.all(np.isfinite(grid2)) and np.all(np.isfinite(w2))
+    assert not np.array_equal(w1, w2) and not np.array_equal(grid1, grid2)

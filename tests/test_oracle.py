@@ -350,9 +350,10 @@ def _unit_cfg(**kw):
 def test_hash_grid_dense_level_reproduces_linear_field(orc, eng, f32):
     """Trilinear interpolation reproduces a linear function exactly: with the
     dense level's entries set to a + b x/N + c y/N + d z/N at vertex (x, y, z)
-    (index x + y (N+1) + z (N+1)^2), the feature at any point u of the unit
-    cube is a + b u_x + c u_y + d u_z.  A swapped weight (f for 1 - f), a
-    transposed index or a wrong stride breaks it."""
+    (entry 8 (bx + B by + B^2 bz) + (x & 1 | (y & 1) << 1 | (z & 1) << 2) of
+    brick (bx, by, bz) = (x, y, z) // 2, B = N/2 + 1), the feature at any
+    point u of the unit cube is a + b u_x + c u_y + d u_z.  A swapped weight
+    (f for 1 - f), a transposed index, a wrong corner bit or stride breaks it."""
     cfg = _unit_cfg(n_levels=2, log2_table_size=19, base_resolution=16, growth=1.5)
     lay = orc.cache_layout(2, 19, 16, 1.5)
     grid = np.zeros(2 * lay["total_entries"], np.float32)
@@ -361,8 +362,11 @@ def test_hash_grid_dense_level_reproduces_linear_field(orc, eng, f32):
         assert lay["dense"][lvl]
         n = lay["res"][lvl]
         s = n + 1
-        idx = np.arange(s ** 3)
-        x, y, z = idx % s, (idx // s) % s, idx // (s * s)
+        v = np.arange(s ** 3)
+        x, y, z = v % s, (v // s) % s, v // (s * s)
+        nb = n // 2 + 1
+        idx = 8 * ((x // 2) + nb * (y // 2) + nb * nb * (z // 2)) + (x % 2) + 2 * (y % 2) + 4 * (z % 2)
+        assert idx.max() < lay["sizes"][lvl]
         for f in range(2):
             a, b, c, d = coef[f] if lvl == 0 else coef[1 - f]
             grid[2 * (lay["offsets"][lvl] + idx) + f] = a + b * x / n + c * y / n + d * z / n
@@ -379,11 +383,13 @@ def test_hash_grid_dense_level_reproduces_linear_field(orc, eng, f32):
 
 @pytest.mark.parametrize("f32", [False, True])
 def test_hash_grid_vertex_values_and_spatial_hash(orc, eng, f32):
-    """At a grid vertex the feature equals that vertex's table entry.  On a
-    hashed level (16 levels from N_min 16, growth 1.5, T = 2^19: level 4 has
-    N = 81, (N+1)^3 > T) the entry of vertex (x, y, z) is
-    (x xor 2654435761 y xor 805459861 z) mod T, checked at hand-picked integer
-    coordinates; on a dense level it is x + y (N+1) + z (N+1)^2."""
+    """At a grid vertex the feature equals that vertex's table entry.  Vertex
+    (x, y, z) is corner (x & 1) | (y & 1) << 1 | (z & 1) << 2 of brick
+    (bx, by, bz) = (x, y, z) // 2, entry 8 brick + corner.  On a hashed level
+    (16 levels from N_min 16, growth 1.5, T = 2^19: level 4 has N = 81,
+    8 (N/2 + 1)^3 > T) brick = (bx xor 2654435761 by xor 805459861 bz) mod T/8,
+    checked at hand-picked integer coordinates; on a dense level brick =
+    bx + B by + B^2 bz with B = N/2 + 1."""
     cfg = _unit_cfg(n_levels=5, log2_table_size=19, base_resolution=16, growth=1.5)
     lay = orc.cache_layout(5, 19, 16, 1.5)
     assert lay["res"][4] == 81 and not lay["dense"][4] and lay["dense"][3]
@@ -398,13 +404,20 @@ def test_hash_grid_vertex_values_and_spatial_hash(orc, eng, f32):
         for k, (x, y, z) in enumerate(verts):
             if lay["dense"][lvl]:
                 x, y, z = (min(c, n) for c in (x, y, z))
-                local = x + y * (n + 1) + z * (n + 1) ** 2
+            corner = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2)
+            bx, by, bz = x >> 1, y >> 1, z >> 1
+            if lay["dense"][lvl]:
+                nb = n // 2 + 1
+                brick = bx + by * nb + bz * nb * nb
             else:
                 m32 = 0xFFFFFFFF
-                local = (x ^ ((y * 2654435761) & m32) ^ ((z * 805459861) & m32)) & (T - 1)
+                brick = (bx ^ ((by * 2654435761) & m32) ^ ((bz * 805459861) & m32)) & (T // 8 - 1)
+            local = 8 * brick + corner
             e = lay["offsets"][lvl] + local
             for f in range(2):
-                assert abs(enc[k, 2 * lvl + f] - grid[2 * e + f]) < (2e-6 if not f32 else 2e-5), (lvl, x, y, z, f)
+                # the point x / N rounds to FP32, so a neighbour enters with a
+                # weight of about 1e-6; a wrong entry is off by O(1)
+                assert abs(enc[k, 2 * lvl + f] - grid[2 * e + f]) < 2e-5, (lvl, x, y, z, f)
 
 
 def _poly_field(res, coef, lo=-1.5, hi=1.5):
