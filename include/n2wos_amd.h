@@ -68,11 +68,16 @@ typedef struct n2wos_bvh n2wos_bvh;
 const char* n2wos_last_error(void);
 int n2wos_abi_version(void);
 /* Launch tuning for benchmarks (process-wide): "ring" = LDS stack entries
- * per lane of the walker (8 or 16, default 8); "refill_min" = idle lanes
+ * per lane of the walker (8 or 16, default 16); "refill_min" = idle lanes
  * that trigger a refill (1..64, default 2); "weight_node" / "weight_tri" /
- * "weight_update" = per-lane weights of the walker's phase choice (10, 10 and
- * 30); "query_quad" = 1 (default): closest-point queries run on four lanes
- * each, 0: one lane per query with the caller's block size (same results);
+ * "weight_update" = per-lane weights of the walker's phase choice (10, 14 for
+ * leaves of up to 3 triangles else 24, and 30); "query_quad" = 1 (default):
+ * closest-point queries run on four lanes each, 0: one lane per query with
+ * the caller's block size (same results); "query_cap" = node visits and
+ * leaves after which a four-lane query stops and a second pass finishes it
+ * with a whole wave (default 16; 0: no second pass);
+ * "query_finish_slots" = consecutive query slots each wave of that pass
+ * scans (1..64, default 8);
  * "spatial_order" = 1 (default): solves of at least 8,192 points claim their
  * work in Morton order of 32^3 cells over the mesh bounds, 0: in the
  * caller's point order; "unit_ranges" = 1..8 (default 8): the claim order is

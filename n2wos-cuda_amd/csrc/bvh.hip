@@ -8,6 +8,7 @@
 
 #include "api_internal.hpp"
 #include "traverse.hpp"
+#include "wave.hpp"
 #include "wos.hpp"
 
 namespace n2 {
@@ -58,12 +59,15 @@ constexpr int kQuadBlock = 256;
 constexpr int kQuadQueries = kQuadBlock / 4;
 constexpr int kQuadCap = 32;  // 16.5 KB per block with the spare slot
 
+// Marker a capped query leaves in its overflow slot for the second pass.
+constexpr int kQueryPending = 2;
+
 template <bool MASKED, bool COUNT>
 __global__ void __launch_bounds__(kQuadBlock) k_closest_point_quad(
     const Bvh4Node* __restrict__ nodes, const TriRec* __restrict__ tris, float delta,
     const float* __restrict__ points, const uint8_t* __restrict__ active, int n, float* __restrict__ out_d2,
     float* __restrict__ out_cp, int* __restrict__ out_tri, int* __restrict__ out_overflow,
-    uint32_t* __restrict__ out_nodes, uint32_t* __restrict__ out_tris) {
+    uint32_t* __restrict__ out_nodes, uint32_t* __restrict__ out_tris, int max_iters) {
   __shared__ uint2 s_ring[(kQuadCap + 1) * kQuadQueries];
   const int t = static_cast<int>(threadIdx.x);
   const int k = t & 3;
@@ -73,18 +77,188 @@ __global__ void __launch_bounds__(kQuadBlock) k_closest_point_quad(
   Hit h;
   if (MASKED && !active[q]) {
     // inactive slot: neutral outputs (src/cuda/cubql_bvh.cu:198-204)
-    h.d2 = 0.0f; h.cp = p; h.id = -1; h.overflow = 0; h.nodes = 0; h.tris = 0;
+    h.d2 = 0.0f; h.cp = p; h.id = -1; h.overflow = 0; h.nodes = 0; h.tris = 0; h.pending = 0;
   } else if (finite3(p)) {
-    h = bvh4_closest_quad<kQuadCap, COUNT>(nodes, tris, p, delta, s_ring + (t >> 2), kQuadQueries, k);
+    h = bvh4_closest_quad<kQuadCap, COUNT>(nodes, tris, p, delta, s_ring + (t >> 2), kQuadQueries, k, max_iters);
   } else {
-    h.d2 = FLT_MAX; h.cp = v3(0.0f, 0.0f, 0.0f); h.id = -1; h.overflow = 0; h.nodes = 0; h.tris = 0;
+    h.d2 = FLT_MAX; h.cp = v3(0.0f, 0.0f, 0.0f); h.id = -1; h.overflow = 0; h.nodes = 0; h.tris = 0; h.pending = 0;
   }
   if (k == 0) {
     out_d2[q] = h.d2;
     out_cp[3 * q] = h.cp.x; out_cp[3 * q + 1] = h.cp.y; out_cp[3 * q + 2] = h.cp.z;
     out_tri[q] = h.id;
-    out_overflow[q] = h.overflow;
+    out_overflow[q] = h.pending ? kQueryPending : h.overflow;
     if (COUNT) { out_nodes[q] = h.nodes; out_tris[q] = h.tris; }
+  }
+}
+
+// Second pass: every query the first pass stopped at its iteration cap
+// (overflow slot = kQueryPending) is finished by a whole wave.  The wave
+// restarts the search from the root with the query's best so far: each round
+// it pops up to 16 entries of a shared LDS stack, one per quad; in one memory
+// phase a quad at an inner node loads child k's box in lane k and a quad at a
+// leaf its first eight records.  Inner quads push the children within the
+// threshold (ballot + rank, no order), leaf quads test their records four at a
+// time (each behind its own box bound), and when some lane holds a (d2, id)
+// below the best, the round's minimum over the wave updates the best and the
+// threshold.  Every entry within the threshold is expanded and every record of
+// an expanded leaf that can still win is tested, so the result is the exact
+// minimum with the lowest index among ties, as in the one-quad search.  Stack
+// keys keep the high 16 bits of their float: a lower bound of the key, so the
+// pruning at a pop stays exact.  The stack holds `cap` entries (dynamic LDS,
+// 6 bytes each), sized from the tree's depth by finish_stack_entries; a push
+// beyond it sets the query's overflow flag.  A block is one wave over `slots`
+// (1..64) consecutive query slots, whose marked queries it finishes one after
+// another; slots with no marked query cost one load.
+//
+// Stack bound: popping up to 16 entries a round and pushing every child
+// within the threshold, the stack of a full traversal with no pruning peaks
+// at 240-276 entries on 65k-triangle trees of depth 8-11 and 352-404 on
+// 1M-triangle trees of depth 13-15 (median and SAH builds, leaves of 3-8),
+// under 28 per level; 64 per level, at least 512 and at most 2048 entries,
+// leaves more than twice that.
+__host__ __device__ constexpr int finish_stack_entries(int depth) {
+  return 64 * (depth + 2) < 512 ? 512 : (64 * (depth + 2) > 2048 ? 2048 : 64 * (depth + 2));
+}
+
+__global__ void __launch_bounds__(64) k_closest_point_finish(const Bvh4Node* __restrict__ nodes,
+                                                             const TriRec* __restrict__ tris, float delta,
+                                                             const float* __restrict__ points, int n,
+                                                             float* __restrict__ out_d2, float* __restrict__ out_cp,
+                                                             int* __restrict__ out_tri,
+                                                             int* __restrict__ out_overflow, int slots, int cap) {
+  extern __shared__ uint32_t s_finish[];
+  uint32_t* const s_ref = s_finish;
+  uint16_t* const s_key = reinterpret_cast<uint16_t*>(s_finish + cap);
+  const int lane = static_cast<int>(threadIdx.x);
+  const int j = lane >> 2, k = lane & 3;
+  const int q0 = static_cast<int>(blockIdx.x) * slots;
+  const bool marked = lane < slots && q0 + lane < n && out_overflow[q0 + lane] == kQueryPending;
+  uint64_t todo = __ballot(marked);
+  while (todo != 0) {
+    const int qq = q0 + __ffsll(static_cast<unsigned long long>(todo)) - 1;
+    todo &= todo - 1;
+    const V3 p = v3(points[3 * qq], points[3 * qq + 1], points[3 * qq + 2]);
+    float bd2 = out_d2[qq];
+    int bid = out_tri[qq];
+    V3 bcp = v3(out_cp[3 * qq], out_cp[3 * qq + 1], out_cp[3 * qq + 2]);
+    float thr = prune_threshold(bd2, delta);
+    int ovf = 0;
+    if (lane == 0) {
+      s_ref[0] = 0u;  // the root
+      s_key[0] = 0;
+    }
+    int sp = 1;  // wave-uniform
+    wave_sync();
+    while (sp > 0) {
+      const int take = sp < 16 ? sp : 16;
+      const int base = sp - take;
+      sp = base;
+      const bool has = j < take;
+      const uint32_t ref = has ? s_ref[base + j] : 0u;
+      const float key = has ? __uint_as_float(static_cast<uint32_t>(s_key[base + j]) << 16) : INFINITY;
+      wave_sync();  // the popped slots may be overwritten by this round's pushes
+      const bool live = has && key <= thr;
+      const bool inner = live && !(ref & kLeafFlag);
+      const int first = static_cast<int>((ref >> 5) & 0x3ffffffu);
+      const int count = live && (ref & kLeafFlag) ? static_cast<int>(ref & 31u) : 0;
+      // the round's one memory phase
+      float lx = 0.0f, ly = 0.0f, lz = 0.0f, hx = 0.0f, hy = 0.0f, hz = 0.0f;
+      uint32_t cr = 0u;
+      if (inner) {
+        const Bvh4Node* nd = nodes + ref;
+        lx = nd->lox[k]; ly = nd->loy[k]; lz = nd->loz[k];
+        hx = nd->hix[k]; hy = nd->hiy[k]; hz = nd->hiz[k];
+        cr = nd->child[k];
+      }
+      float4 t0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), t1 = t0, t2 = t0, u0 = t0, u1 = t0, u2 = t0;
+      if (k < count) {
+        const float4* tp = reinterpret_cast<const float4*>(tris + first + k);
+        t0 = tp[0]; t1 = tp[1]; t2 = tp[2];
+      }
+      if (4 + k < count) {
+        const float4* tp = reinterpret_cast<const float4*>(tris + first + 4 + k);
+        u0 = tp[0]; u1 = tp[1]; u2 = tp[2];
+      }
+      const float ck = inner ? box_dist2(p, lx, ly, lz, hx, hy, hz) : INFINITY;
+      const bool push = inner && ck <= thr;
+      const uint64_t pm = __ballot(push);
+      const int npush = __popcll(pm);
+      if (sp + npush > cap) {
+        ovf = 1;  // the stack is full: the query reports overflow
+      } else {
+        if (push) {
+          const int slot = sp + rank_in(pm);
+          s_ref[slot] = cr;
+          s_key[slot] = static_cast<uint16_t>(__float_as_uint(ck) >> 16);
+        }
+        sp += npush;
+      }
+      // leaves: records first + r0 + k, four per quad at a time
+      float td2 = INFINITY;
+      int tid = 0x7fffffff;
+      int tlane_rec = -1;
+      for (int r0 = 0;; r0 += 4) {
+        const bool test = r0 + k < count;
+        if (__ballot(test) == 0) break;
+        if (test) {
+          if (r0 == 4) {
+            t0 = u0; t1 = u1; t2 = u2;
+          } else if (r0 > 4) {
+            const float4* tp = reinterpret_cast<const float4*>(tris + first + r0 + k);
+            t0 = tp[0]; t1 = tp[1]; t2 = tp[2];
+          }
+          // the triangle's own box bounds its distance from below
+          const float bxl = fminf(fminf(t0.x, t0.w), t1.z), bxh = fmaxf(fmaxf(t0.x, t0.w), t1.z);
+          const float byl = fminf(fminf(t0.y, t1.x), t1.w), byh = fmaxf(fmaxf(t0.y, t1.x), t1.w);
+          const float bzl = fminf(fminf(t0.z, t1.y), t2.x), bzh = fmaxf(fmaxf(t0.z, t1.y), t2.x);
+          if (box_dist2(p, bxl, byl, bzl, bxh, byh, bzh) <= thr) {
+            const V3 a = v3(t0.x, t0.y, t0.z), b = v3(t0.w, t1.x, t1.y), c = v3(t1.z, t1.w, t2.x);
+            const V3 cq = closest_on_triangle_flat(p, a, b, c);
+            const V3 e = cq - p;
+            const float d2 = dot(e, e);
+            const int id = __float_as_int(t2.y);
+            if (d2 < td2 || (d2 == td2 && id < tid)) {
+              td2 = d2;
+              tid = id;
+              tlane_rec = first + r0 + k;
+            }
+          }
+        }
+      }
+      if (__ballot(td2 < bd2 || (td2 == bd2 && tid < bid)) != 0) {
+        // the round's (d2, id) minimum over the wave: d2 >= 0, so its bits
+        // order like its values, and the id breaks ties
+        uint64_t key64 = (static_cast<uint64_t>(__float_as_uint(td2)) << 32) | static_cast<uint32_t>(tid);
+        for (int off = 32; off > 0; off >>= 1) {
+          const uint64_t o =
+              (static_cast<uint64_t>(lane_read(static_cast<uint32_t>(key64 >> 32), lane ^ off)) << 32) |
+              lane_read(static_cast<uint32_t>(key64), lane ^ off);
+          key64 = o < key64 ? o : key64;
+        }
+        const float rd2 = __uint_as_float(static_cast<uint32_t>(key64 >> 32));
+        const int rid = static_cast<int>(static_cast<uint32_t>(key64));
+        // the winner's lane recomputes its closest point (the same operations)
+        const uint64_t wm = __ballot(tlane_rec >= 0 && td2 == rd2 && tid == rid);
+        const int wl = __ffsll(static_cast<unsigned long long>(wm)) - 1;
+        const int rec = static_cast<int>(lane_read(static_cast<uint32_t>(tlane_rec), wl));
+        const TriRec& tr = tris[rec];
+        const V3 a = v3(tr.ax, tr.ay, tr.az), b = v3(tr.bx, tr.by, tr.bz), c = v3(tr.cx, tr.cy, tr.cz);
+        bcp = closest_on_triangle_flat(p, a, b, c);
+        bd2 = rd2;
+        bid = rid;
+        thr = fminf(thr, prune_threshold(bd2, delta));
+      }
+      wave_sync();
+    }
+    if (lane == 0) {
+      out_d2[qq] = bd2;
+      out_cp[3 * qq] = bcp.x;
+      out_cp[3 * qq + 1] = bcp.y;
+      out_cp[3 * qq + 2] = bcp.z;
+      out_tri[qq] = bid;
+      out_overflow[qq] = ovf;
+    }
   }
 }
 
@@ -167,9 +341,20 @@ int launch_query(const n2wos_bvh* bvh, const float* d_points, const uint8_t* d_a
     // above as the reference does, changes no result
     const int64_t lanes = 4 * static_cast<int64_t>(n);
     const int grid = static_cast<int>((lanes + kQuadBlock - 1) / kQuadBlock);
+    // queries past the cap are finished by the wave-wide second pass (the
+    // counting build runs every query to its end in one pass)
+    const int cap = COUNT ? 0 : tuning_value("query_cap", 16);
+    const int fslots = tuning_value("query_finish_slots", 8);
     hipLaunchKernelGGL((k_closest_point_quad<MASKED, COUNT>), dim3(grid), dim3(kQuadBlock), 0,
                        static_cast<hipStream_t>(stream), bvh->d_nodes, bvh->d_tris, bvh->delta, d_points, d_active, n,
-                       d_d2, d_cp, d_tri, d_overflow, d_nodes, d_tris);
+                       d_d2, d_cp, d_tri, d_overflow, d_nodes, d_tris, cap);
+    if (cap > 0) {
+      const int scap = finish_stack_entries(bvh->depth);
+      hipLaunchKernelGGL(k_closest_point_finish, dim3((n + fslots - 1) / fslots), dim3(64),
+                         static_cast<size_t>(scap) * (sizeof(uint32_t) + sizeof(uint16_t)),
+                         static_cast<hipStream_t>(stream), bvh->d_nodes, bvh->d_tris, bvh->delta, d_points, n, d_d2,
+                         d_cp, d_tri, d_overflow, fslots, scap);
+    }
     N2_HIP(hipGetLastError());
     return N2WOS_OK;
   }

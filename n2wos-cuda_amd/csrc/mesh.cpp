@@ -111,8 +111,12 @@ extern "C" int n2wos_set_tuning(const char* key, int value) {
     N2_REQUIRE(key != nullptr, N2WOS_ERR_NULL, "NULL key");
     const std::string k(key);
     N2_REQUIRE(k == "ring" || k == "refill_min" || k == "weight_node" || k == "weight_tri" || k == "weight_update" ||
-                   k == "query_quad" || k == "unit_ranges" || k == "spatial_order",
+                   k == "query_quad" || k == "unit_ranges" || k == "spatial_order" || k == "query_cap" ||
+                   k == "query_finish_slots",
                N2WOS_ERR_INVALID, "unknown tuning key: " + k);
+    if (k == "query_cap") N2_REQUIRE(value >= 0, N2WOS_ERR_INVALID, "query_cap must be >= 0 (0: no second pass)");
+    if (k == "query_finish_slots")
+      N2_REQUIRE(value >= 1 && value <= 64, N2WOS_ERR_INVALID, "query_finish_slots must be in [1, 64]");
     if (k == "spatial_order") N2_REQUIRE(value == 0 || value == 1, N2WOS_ERR_INVALID, "spatial_order must be 0 or 1");
     if (k == "unit_ranges") N2_REQUIRE(value >= 1 && value <= 8, N2WOS_ERR_INVALID, "unit_ranges must be in [1, 8]");
     if (k == "ring") N2_REQUIRE(value == 8 || value == 16, N2WOS_ERR_INVALID, "ring must be 8 or 16");

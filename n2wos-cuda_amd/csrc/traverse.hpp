@@ -22,6 +22,8 @@
 #include "bvh.hpp"
 #include "geom.hpp"
 
+#include <climits>
+
 namespace n2 {
 
 struct Hit {
@@ -31,6 +33,7 @@ struct Hit {
   int overflow;
   uint32_t nodes;  // node lines loaded (counting variant only)
   uint32_t tris;   // triangle records tested (counting variant only)
+  int pending;     // stopped at the iteration cap with work left (d2, cp, id: the best so far)
 };
 
 __device__ __forceinline__ void cswap(float& ka, uint32_t& ra, int& sa, float& kb, uint32_t& rb, int& sb) {
@@ -58,6 +61,7 @@ __device__ __forceinline__ Hit bvh4_closest(const Bvh4Node* __restrict__ nodes, 
   h.overflow = 0;
   h.nodes = 0;
   h.tris = 0;
+  h.pending = 0;
   uint64_t spill_e[kSpill];  // ref | distance bits << 32
   float thr = thr_init;
   uint32_t cur = 0;
@@ -223,10 +227,15 @@ __device__ __forceinline__ void quad_min_step(float& d2, int& id, V3& q) {
   q = v3(take ? ox : q.x, take ? oy : q.y, take ? oz : q.z);
 }
 
+// max_iters > 0 stops the query at its first pop after at least that many
+// node visits and leaves (at most one descent past it) with work left
+// (h.pending = 1, d2 / cp / id the best so far); <= 0: no limit.  The cap is
+// tested only where the loop already decides to end, so uncapped queries run
+// the same loop.
 template <int CAP, bool COUNT>
 __device__ __forceinline__ Hit bvh4_closest_quad(const Bvh4Node* __restrict__ nodes,
                                                  const TriRec* __restrict__ tris, V3 p, float delta, uint2* sent,
-                                                 int stride, int k) {
+                                                 int stride, int k, int max_iters) {
   static_assert((CAP & (CAP - 1)) == 0, "ring capacity must be a power of two");
   Hit h;
   h.d2 = FLT_MAX;
@@ -235,13 +244,17 @@ __device__ __forceinline__ Hit bvh4_closest_quad(const Bvh4Node* __restrict__ no
   h.overflow = 0;
   h.nodes = 0;
   h.tris = 0;
+  h.pending = 0;
   uint64_t spill_e[kSpill];  // ref | distance bits << 32
   float thr = FLT_MAX;
   uint32_t cur = 0;
   bool have = true;
   int top = 0, cnt = 0, scnt = 0;
+  int iters = 0;
+  const int lim = max_iters > 0 ? max_iters : INT_MAX;
   while (true) {
     if (have) {
+      ++iters;
       if (!(cur & kLeafFlag)) {
         const Bvh4Node* nd = nodes + cur;
         if (COUNT) ++h.nodes;
@@ -349,7 +362,9 @@ __device__ __forceinline__ Hit bvh4_closest_quad(const Bvh4Node* __restrict__ no
         break;
       }
     }
-    if (!have) break;
+    const bool stop = have && iters >= lim;
+    if (stop) h.pending = 1;
+    if (!have || stop) break;
   }
   return h;
 }

@@ -182,3 +182,35 @@ def test_host_vector_query_equals_device_query(eng, torch_gpu, c1):
     with pytest.raises(eng.N2wosError) as err:
         bvh.query_host(pts[:4], block_size=0)
     assert err.value.code == eng._lib.ERR_BLOCK
+
+
+@pytest.mark.parametrize("cap,slots", [(1, 64), (4, 16), (4, 7), (32, 1)])
+@pytest.mark.parametrize("method,leaf", [("median", 8), ("sah", 3)])
+def test_capped_queries_finished_by_the_wave_pass_are_exact(eng, orc, torch_gpu, cap, slots, method, leaf):
+    """The query kernel stops a query after `query_cap` node visits and
+    leaves, and a second pass finishes it with a whole wave (each wave
+    scanning `slots` consecutive queries; 4,200 points leave a partial last
+    run for 16 and 64 slots).  With tiny caps nearly every query goes through that pass: the
+    results still equal the brute force bit for bit (lowest index among
+    ties), with no overflow, on uniform, near-surface, vertex, far and centre
+    points."""
+    from n2wos_amd import _lib
+    v, t = eng.uv_sphere(64, 128, 1, [0.1])
+    vn, _, _ = eng.normalize_to_unit_radius(v)
+    bvh = eng.Bvh(vn, t, leaf_size=leaf, build_method=method)
+    rng = np.random.default_rng(7 + cap)
+    base = rng.uniform(-1.2, 1.2, (3000, 3)).astype(np.float32)
+    _, cpb, _ = orc.bruteforce(vn, t, base[:500], nthreads=16)
+    pts = np.concatenate([base, cpb + rng.normal(0, 1e-5, cpb.shape), vn[:200], rng.uniform(-0.05, 0.05, (300, 3)),
+                          rng.uniform(-20, 20, (100, 3))]).astype(np.float32)
+    print("case: capped queries", cap, slots, method, leaf, len(pts), flush=True)
+    _lib.lib().n2wos_set_tuning(b"query_cap", cap)
+    _lib.lib().n2wos_set_tuning(b"query_finish_slots", slots)
+    try:
+        d2, cp, tri, ovf = bvh.query(pts)
+    finally:
+        _lib.lib().n2wos_set_tuning(b"query_cap", 16)
+        _lib.lib().n2wos_set_tuning(b"query_finish_slots", 8)
+    d2b, cpb2, trb = orc.bruteforce(vn, t, pts, nthreads=16)
+    assert np.array_equal(bits(d2), bits(d2b)) and np.array_equal(tri, trb) and np.array_equal(bits(cp), bits(cpb2))
+    assert ovf.sum() == 0
