@@ -238,8 +238,9 @@ typedef struct {
 
 /* Bytes of device workspace n2wos_wos_solve needs for this job on the current
  * device (a negative error code if the device cannot be queried): per-unit
- * and per-point state plus a staging area for walk values whose size follows
- * the device's resident walker blocks (at most 10.5 MB on MI355X). */
+ * and per-point state plus a staging area for walk values and a 128-byte
+ * line per walker lane for traversal-stack spills, whose size follows the
+ * device's resident walker blocks (at most 52.4 MB on MI355X). */
 int64_t n2wos_wos_workspace_size(int n_points, int walks_per_point);
 /* Blocks of the persistent walker's full-device grid on the current device
  * for this source term (the launch that max_blocks = 0 asks for), so that a
@@ -262,9 +263,9 @@ int n2wos_wos_solve(const n2wos_bvh* bvh, const n2wos_boundary* boundary, const 
                     void* stream);
 
 /* Largest walker workspace (n2wos_wos_workspace_size, bytes) that
- * n2wos_wos_solve_noalloc serves: 32 MiB.  For example 170,132 points x 256
- * walks need 32.5 MB on MI355X; 340,000 points x 256 walks do not fit. */
-#define N2WOS_NOALLOC_SCRATCH_BYTES (32LL << 20)
+ * n2wos_wos_solve_noalloc serves: 96 MiB.  For example 170,132 points x 256
+ * walks need 74.5 MB on MI355X; 400,000 points x 256 walks do not fit. */
+#define N2WOS_NOALLOC_SCRATCH_BYTES (96LL << 20)
 /* n2wos_wos_solve without the workspace arguments, for callers that cannot
  * allocate device memory: code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc.  No counterpart in the reference.  The

@@ -100,6 +100,14 @@ constexpr int kUpdate = 3;  // query finished: take the walk step (or end the wa
 // (a leaf of more is finished in later iterations).
 constexpr int kTriTake = 4;
 
+// Ring overflow goes first to the lane's own line of kLineSpill entries in the
+// workspace (one 128-byte line per lane, so a lane's spills and the loads
+// that take them back touch one line, which stays in L2 between them), then
+// to the per-lane scratch array of kSpill entries.  Per-lane scratch is
+// interleaved across the wave's lanes dword by dword: an 8-byte entry there
+// dirties two lines, each holding 4 bytes of it.
+constexpr int kLineSpill = 16;
+
 // Walk flags beside the unit slot and walk-in-unit index (bits 0-8).
 constexpr int kFlagBounded = 1 << 9;  // the query started from the step bound, not unbounded
 constexpr int kFlagOvf = 1 << 10;     // the query's traversal stack overflowed
@@ -172,6 +180,11 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
   uint64_t spill_e[kSpill];
   uint32_t n_nodes = 0, n_tris = 0;
 
+  // this lane's spill line, computed where it is used (spills are rare): the
+  // opaque lane index keeps its address out of the registers the loop holds
+#define N2_SPILL_LINE()                                                                         \
+  (KA->spill_lines + (static_cast<size_t>(blockIdx.x) * kBlock + static_cast<size_t>(wv) * 64 + \
+                      static_cast<uint32_t>(opaque_lane_int(lane_id()))) * kLineSpill)
   // go to the entry `r` of the traversal
 #define N2_ENTER(r)                                       \
   do {                                                    \
@@ -200,7 +213,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
         pop_ref_ = sref[top * kBlock];               \
       } else {                                       \
         --scnt;                                      \
-        const uint64_t se_ = spill_e[scnt];          \
+        const uint64_t se_ = scnt < kLineSpill ? N2_SPILL_LINE()[scnt] : spill_e[scnt - kLineSpill]; \
         pop_key_ = __uint_as_float(static_cast<uint32_t>(se_ >> 32)); \
         pop_ref_ = static_cast<uint32_t>(se_);       \
       }                                              \
@@ -221,9 +234,13 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
     const int m_ = (M);                                                           \
     if (cnt + m_ > CAP) {                                                         \
       for (int j_ = CAP - cnt; j_ < m_; ++j_) {                                   \
-        if (scnt < kSpill) {                                                      \
+        if (scnt < kLineSpill + kSpill) {                                         \
           const int vs_ = ((top + j_) & (CAP - 1)) * kBlock;                       \
-          spill_e[scnt] = (static_cast<uint64_t>(skey[vs_]) << 48) | sref[vs_];    \
+          const uint64_t e_ = (static_cast<uint64_t>(skey[vs_]) << 48) | sref[vs_]; \
+          if (scnt < kLineSpill)                                                  \
+            N2_SPILL_LINE()[scnt] = e_;                                           \
+          else                                                                    \
+            spill_e[scnt - kLineSpill] = e_;                                      \
           ++scnt;                                                                 \
         } else {                                                                  \
           wflags |= kFlagOvf;                                                     \
@@ -677,6 +694,7 @@ __global__ void __launch_bounds__(kBlock, N2_WALKER_WAVES_PER_SIMD) k_wos_persis
 #undef N2_ENTER
 #undef N2_POP_NEXT
 #undef N2_PUSH3
+#undef N2_SPILL_LINE
 #undef N2_NODE_VISIT
   if (COUNT) {
     atomicAdd(reinterpret_cast<unsigned long long*>(a.visits), static_cast<unsigned long long>(n_nodes));

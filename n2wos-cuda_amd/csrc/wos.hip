@@ -271,10 +271,13 @@ int64_t wos_workspace_bytes(int n_points, int walks_per_point) {
   // + first query per point: d2, closest xyz, tri, overflow (24 B), 256-aligned
   // + claim order: cell and order per point (8 B), cell counters (2 x 128 KiB)
   // + unit-value staging: 256 B per unit slot of every block the launch may have
+  // + spill lines: 128 B per lane of those blocks
   const int64_t first = (static_cast<int64_t>(n_points) * 24 + 255) / 256 * 256;
   const int64_t order = (static_cast<int64_t>(n_points) * 8 + 255) / 256 * 256 + 2 * 4 * kOrderCells;
-  const int64_t staging = staging_blocks(n_points, walks_per_point) * kWaves * kSlots * 64 * 4;
-  return 256 + (units * 24 + 255) / 256 * 256 + first + order + staging;
+  const int64_t blocks = staging_blocks(n_points, walks_per_point);
+  const int64_t staging = blocks * kWaves * kSlots * 64 * 4;
+  const int64_t lines = blocks * kBlock * kLineSpill * 8;
+  return 256 + (units * 24 + 255) / 256 * 256 + first + order + staging + lines;
 }
 
 void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStream_t stream) {
@@ -328,6 +331,8 @@ void run_wos(const n2wos_bvh* bvh, const Boundary& bc, const WosRun& r, hipStrea
   uint32_t* order_count = reinterpret_cast<uint32_t*>(ord + (static_cast<int64_t>(r.n_points) * 8 + 255) / 256 * 256);
   uint32_t* order_cursor = order_count + kOrderCells;
   a.unit_vals = reinterpret_cast<float*>(reinterpret_cast<char*>(order_count) + 2 * 4 * kOrderCells);
+  a.spill_lines = reinterpret_cast<uint64_t*>(
+      a.unit_vals + staging_blocks(r.n_points, r.walks_per_point) * kWaves * kSlots * 64);
   a.order = nullptr;
   if (r.n_points >= kOrderMinPoints && tuning_value("spatial_order", 1) != 0) {
     // claim order by cell of the mesh bounds (the points themselves may lie outside)

@@ -54,6 +54,10 @@ struct WosKernelArgs {
   // walk values staged per unit slot until the unit's last walk ends:
   // [block][wave][slot][64 walks in the unit]
   float* unit_vals;
+  // traversal-stack entries spilled from the LDS ring: kLineSpill per lane,
+  // lane-major ([block][wave][lane][kLineSpill], 8 B each), so one lane's
+  // spills share one 128-byte line
+  uint64_t* spill_lines;
   double* unit_mean;
   double* unit_m2;
   uint32_t* unit_steps;

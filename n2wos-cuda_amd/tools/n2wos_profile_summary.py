@@ -107,7 +107,7 @@ def main(argv=None):
     ap.add_argument("--occ-dir")
     ap.add_argument("--walk-steps", type=float)
     ap.add_argument("--kernel-ms", type=float)
-    ap.add_argument("--workload", default="benchmark problem 2, SAH BVH leaf 3, 170,132 points x 256 walks")
+    ap.add_argument("--workload", default="benchmark problem 2, SAH BVH leaf 4, 170,132 points x 256 walks")
     ap.add_argument("--command", action="append", default=[], help="a profiling command, recorded verbatim")
     ap.add_argument("--output", required=True)
     a = ap.parse_args(argv)

@@ -235,6 +235,8 @@ extern "C" int emu_walker(const void* nodes, const void* tris, float delta, cons
   a.order = n2::emu_order;
   static float unit_vals[8 * 64];  // staging of the one emulated wave's 8 unit slots
   a.unit_vals = unit_vals;
+  static uint64_t spill_lines[64 * n2::kLineSpill];  // the one emulated wave's spill lines
+  a.spill_lines = spill_lines;
   a.unit_mean = unit_mean;
   a.unit_m2 = unit_m2;
   a.unit_steps = unit_steps;
