@@ -233,3 +233,39 @@ def test_exterior_start_points_bit_exact(eng, orc, torch_gpu, mesh):
     assert np.array_equal(gs, os_)
     assert np.array_equal(bits(gv), bits(ov))
     assert (gs[:3 * 64] == 256).mean() > 0.5  # most exterior walks run to the cap
+
+
+def test_ring_capacity_and_spill_lines_change_nothing(eng, orc, torch_gpu):
+    """The walker's LDS ring overflows to a 128-byte line per lane in the
+    workspace and then to per-lane scratch.  On mesh 2 with the benchmark's
+    SAH leaf-4 tree, the 8-entry ring (about two spills per walk step; on the
+    CPU emulator about one step in 40,000 spills past the line into scratch)
+    and the default 16-entry ring give the same
+    per-walk values, steps and FP64 statistics bit for bit, and a sample of
+    points equals the oracle's walks."""
+    from n2wos_amd import _lib as L, problems as P, wos as W
+    torch = torch_gpu
+    v, t = P.make_mesh("mesh2")
+    bvh = eng.Bvh(v, t, leaf_size=4, build_method="sah")
+    pts = P.interior_points(bvh, v, 3000, seed=31)
+    bc = P.charges_boundary(0)
+    wp = W.WalkParams(epsilon=1e-4, seed=12)
+    runs = []
+    try:
+        for cap in (16, 8):
+            print("case: ring %d, mesh2 3000x64" % cap, flush=True)
+            L.check(L.lib().n2wos_set_tuning(b"ring", cap))
+            runs.append(run(torch, bvh, bc, wp, pts, 64))
+    finally:
+        L.check(L.lib().n2wos_set_tuning(b"ring", 16))
+    a, b = runs
+    assert np.array_equal(bits(a.extra["walk_values"].cpu().numpy()), bits(b.extra["walk_values"].cpu().numpy()))
+    assert np.array_equal(a.extra["walk_steps"].cpu().numpy(), b.extra["walk_steps"].cpu().numpy())
+    for f in ("n", "mean", "m2"):
+        assert np.array_equal(getattr(a, f).cpu().numpy().view(np.uint64), getattr(b, f).cpu().numpy().view(np.uint64))
+    assert int(a.forced.sum().item()) == int(b.forced.sum().item())
+    sel = np.arange(0, 3000, 100)
+    ov, os_, _, _ = orc.Bvh2(v, t).walks(pts[sel], 64, max_steps=256, eps=1e-4, step_scale=0.999, bmode=bc.mode,
+                                         bparams=bc.oracle_params(), seed=12, nthreads=16, point_ids=sel)
+    gv = a.extra["walk_values"].cpu().numpy().reshape(3000, 64)[sel].ravel()
+    assert np.array_equal(bits(gv), bits(ov))
