@@ -35,6 +35,15 @@ __device__ __forceinline__ f4_t mfma16(h4_t a, h4_t b, f4_t c) {
   return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
 }
 
+// v_mfma_f32_16x16x32_f16 (gfx950): the same fragments with K = 32, lane
+// (i, g) holding A[i][8g .. 8g+7] and B[8g .. 8g+7][i], the accumulator as
+// above.  Used where K is summed over as a whole (the weight gradients), so
+// only the pairing of A's and B's K elements matters.
+typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f4_t mfma32(h8_t a, h8_t b, f4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
 __device__ __forceinline__ h4_t relu_h4(f4_t v) {
   h4_t r;
   r[0] = static_cast<_Float16>(fmaxf(v[0], 0.0f));
@@ -123,6 +132,7 @@ __device__ __forceinline__ void encode_fragments(const GridLayout& gl, const flo
 }
 
 __device__ __forceinline__ h4_t load_h4(const _Float16* p) { return *reinterpret_cast<const h4_t*>(p); }
+__device__ __forceinline__ h8_t load_h8(const _Float16* p) { return *reinterpret_cast<const h8_t*>(p); }
 
 // A fragment of a row-major [rows x cols] FP16 matrix: row r0 + i, cols k0 + 4g ..
 __device__ __forceinline__ h4_t frag_rows(const _Float16* m, int cols, int r, int k) {
@@ -426,9 +436,10 @@ __global__ void __launch_bounds__(kCacheBlock) k_cache_train_pass1(GridLayout gl
 // Training pass 2: weight gradients dA = dZ H_prev^T over the batch (K =
 // samples) on MFMA.  blockIdx.y = layer (0 = input layer, 1 .. L-1 hidden,
 // L = output); each wave reduces a K-chunk into ROWS_T x COLS_T accumulator
-// tiles and adds them to the FP32 gradient with atomics.  ROWS_T = 0 marks
-// the output layer, whose dZ is the single row dY (rows 1..15 of the padded
-// tile are zero).
+// tiles, 32 samples per MFMA (16-byte loads; a chunk's last 16 samples, when
+// the batch is an odd multiple of 16, with the 16-deep MFMA), and adds them
+// to the FP32 gradient with atomics.  ROWS_T = 0 marks the output layer,
+// whose dZ is the single row dY (rows 1..15 of the padded tile are zero).
 template <int ROWS_T, int COLS_T>
 __device__ __forceinline__ void wgrad_layer(const _Float16* __restrict__ dz, const _Float16* __restrict__ hp,
                                             size_t B, int k0, int k1, int i, int g, int cols, float* __restrict__ wg,
@@ -437,7 +448,26 @@ __device__ __forceinline__ void wgrad_layer(const _Float16* __restrict__ dz, con
   f4_t acc[R * COLS_T];
 #pragma unroll
   for (int q = 0; q < R * COLS_T; ++q) acc[q] = zero4();
-  for (int k = k0; k < k1; k += 16) {
+  int k = k0;
+  for (; k + 32 <= k1; k += 32) {
+    h8_t bf[COLS_T];
+#pragma unroll
+    for (int c = 0; c < COLS_T; ++c) bf[c] = load_h8(hp + static_cast<size_t>(16 * c + i) * B + k + 8 * g);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      h8_t af;
+      if (ROWS_T == 0) {
+        const h8_t v = load_h8(dz + k + 8 * g);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) af[e] = i == 0 ? v[e] : static_cast<_Float16>(0.0f);
+      } else {
+        af = load_h8(dz + static_cast<size_t>(16 * r + i) * B + k + 8 * g);
+      }
+#pragma unroll
+      for (int c = 0; c < COLS_T; ++c) acc[r * COLS_T + c] = mfma32(af, bf[c], acc[r * COLS_T + c]);
+    }
+  }
+  if (k < k1) {  // 16 samples left
     h4_t bf[COLS_T];
 #pragma unroll
     for (int c = 0; c < COLS_T; ++c) bf[c] = load_h4(hp + static_cast<size_t>(16 * c + i) * B + k + 4 * g);
