@@ -214,3 +214,49 @@ def test_capped_queries_finished_by_the_wave_pass_are_exact(eng, orc, torch_gpu,
     d2b, cpb2, trb = orc.bruteforce(vn, t, pts, nthreads=16)
     assert np.array_equal(bits(d2), bits(d2b)) and np.array_equal(tri, trb) and np.array_equal(bits(cp), bits(cpb2))
     assert ovf.sum() == 0
+
+
+def test_two_pass_query_replays_in_a_hip_graph(eng, orc, torch_gpu, c1):
+    """The default query (the four-lane pass, then the wave-wide pass for the
+    queries it stopped) captured in a HIP graph: three replays, each after
+    eager work that overwrites the outputs, equal the eager call and the brute
+    force bit for bit.  Uniform and centre points, so most queries reach the
+    second pass."""
+    from n2wos_amd._lib import check, lib
+    torch = torch_gpu
+    vn, t, bvh = c1
+    rng = np.random.default_rng(41)
+    pts = np.concatenate([rng.uniform(-1.2, 1.2, (3000, 3)), rng.uniform(-0.05, 0.05, (1000, 3))]).astype(np.float32)
+    n = len(pts)
+    pt = torch.from_numpy(pts).cuda()
+    d2 = torch.empty(n, dtype=torch.float32, device="cuda")
+    cp = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+    tri = torch.empty(n, dtype=torch.int32, device="cuda")
+    ovf = torch.empty(n, dtype=torch.int32, device="cuda")
+
+    def call():
+        check(lib().n2wos_bvh_query_device(bvh.handle, pt.data_ptr(), n, d2.data_ptr(), cp.data_ptr(), tri.data_ptr(),
+                                           ovf.data_ptr(), 128, torch.cuda.current_stream().cuda_stream))
+
+    print("case: two-pass query in a HIP graph, %d points" % n, flush=True)
+    call()
+    torch.cuda.synchronize()
+    ref = [x.cpu().numpy().copy() for x in (d2, cp, tri, ovf)]
+    d2b, cpb, trb = orc.bruteforce(vn, t, pts, nthreads=16)
+    assert np.array_equal(bits(ref[0]), bits(d2b)) and np.array_equal(ref[2], trb)
+    assert np.array_equal(bits(ref[1]), bits(cpb)) and ref[3].sum() == 0
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        call()
+    for rep in range(3):
+        d2.fill_(-1.0)
+        cp.fill_(7.0)
+        tri.fill_(-7)
+        ovf.fill_(5)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        got = [x.cpu().numpy() for x in (d2, cp, tri, ovf)]
+        assert np.array_equal(bits(got[0]), bits(ref[0])), rep
+        assert np.array_equal(bits(got[1]), bits(ref[1])), rep
+        assert np.array_equal(got[2], ref[2]) and np.array_equal(got[3], ref[3]), rep
