@@ -108,7 +108,7 @@ class CvSolver:
 
     def solve(self, points, wpp: int, field: CvField | None = None, stats: PointStats | None = None,
               accumulate: bool = False, point_offset: int = 0, per_walk: bool = False, path_len: int = 0,
-              seed: int | None = None, stream_id: int | None = None, events=None):
+              seed: int | None = None, stream_id: int | None = None, events=None, max_steps: int | None = None):
         import torch
         n = points.shape[0]
         dev = points.device
@@ -126,6 +126,8 @@ class CvSolver:
         wp = self.params.to_c()
         if seed is not None:
             wp.seed = seed
+        if max_steps is not None:
+            wp.max_steps = max_steps
         if stream_id is not None:
             wp.rng_stream = stream_id
         if events is not None:
@@ -162,14 +164,18 @@ class CvSolver:
 
 def train_field(bvh, verts, boundary, params, points, cache_kw=None, label_stride: int = 2, path_len: int = 32,
                 rounds: int = 1, train_steps=(40, 40), batch: int = 1 << 15, res: int = 64, design: int = 6,
-                seed: int = 11, solver: "CvSolver | None" = None):
+                seed: int = 11, solver: "CvSolver | None" = None, label_max_steps: int | None = None):
     """The training pipeline of the control variate: labels from the path
     records of one plain walk per labelled point, Adam steps, a bake of the
     cache onto a res^3 grid over the padded mesh bounds, then `rounds`
     bootstrap rounds (labels from control-variate walks with the current
     field, far less noisy, and more Adam steps).  points: a CUDA (n, 3)
     tensor of interior points (every label_stride-th is labelled).
-    Returns (field, cache, labels per round)."""
+    label_max_steps caps the label walks only (a capped walk ends on the
+    boundary value at its closest point, as at max_steps): the labels then
+    carry a small bias, which the estimator does not inherit, since the
+    correction has zero mean for any field, while the label solves no longer
+    wait for the longest walks.  Returns (field, cache, labels per round)."""
     import torch
 
     from . import cache as K
@@ -177,7 +183,7 @@ def train_field(bvh, verts, boundary, params, points, cache_kw=None, label_strid
     cs = solver or CvSolver(bvh, boundary, params)
     cache = K.Cache(K.CacheConfig.for_bounds(lo, hi, seed=seed, max_batch=batch, **(cache_kw or {})))
     lp = points[::label_stride].contiguous() if label_stride > 1 else points
-    st = cs.solve(lp, 1, None, path_len=path_len, seed=seed + 101, stream_id=5)
+    st = cs.solve(lp, 1, None, path_len=path_len, seed=seed + 101, stream_id=5, max_steps=label_max_steps)
     x, y = CvSolver.path_labels(st)
     cache.set_output_offset(float(y.mean().item()))
     gen = torch.Generator(device="cuda").manual_seed(seed)
@@ -190,7 +196,8 @@ def train_field(bvh, verts, boundary, params, points, cache_kw=None, label_strid
             cache.train_step(x[idx].contiguous(), y[idx].contiguous())
         fld = bake(cache, lo, hi, res=res, points=design)
         if r < rounds:
-            st = cs.solve(lp, 1, fld, path_len=path_len, seed=seed + 200 + r, stream_id=6 + r)
+            st = cs.solve(lp, 1, fld, path_len=path_len, seed=seed + 200 + r, stream_id=6 + r,
+                          max_steps=label_max_steps)
             x, y = CvSolver.path_labels(st)
     return fld, cache, nlab
 

@@ -131,3 +131,50 @@ def test_cv_estimator_unbiased_with_trained_cache(eng, torch_gpu):
     assert abs(me) < 5 * se
     assert np.mean(np.abs(z) > 4) < 0.01 and 0.85 < z.std() < 1.15
     assert var.mean() < vp / 5
+
+
+def test_cv_estimator_unbiased_with_capped_label_walks(eng, torch_gpu):
+    """The training pipeline with its label walks capped at 48 steps (a cap
+    that forces about one in twelve of them, so the labels are biased): the cap
+    reaches the label walks only, and the estimate over the resulting field,
+    16 walks per point at the full 256-step limit on every 4th grid point of
+    problem 2, stays unbiased against the FP64 solution (mean error within 5
+    standard errors, calibrated z-scores) with a walk variance still at least
+    5x below plain WoS's."""
+    from n2wos_amd import cv as CV, problems as P, wos as W
+    torch = torch_gpu
+    v, t = P.make_mesh("mesh2")
+    bvh = eng.Bvh(v, t, leaf_size=4, build_method="sah")
+    pts, _ = P.grid_slice_points(bvh, 512)
+    pts = pts[::4].copy()
+    tp = torch.from_numpy(pts).cuda()
+    bc = P.charges_boundary(0)
+    truth = P.charges_solution_f64(bc.charges, pts)
+    wp = W.WalkParams(epsilon=1e-4, seed=12345)
+    cs = CV.CvSolver(bvh, bc, wp)
+    print("case: cv capped labels, %d points" % len(pts), flush=True)
+    lab = cs.solve(tp[::2].contiguous(), 1, None, path_len=32, seed=5, stream_id=5, max_steps=48)
+    steps = lab.extra["walk_steps"].cpu().numpy()
+    forced = int(lab.forced.sum().item())
+    assert steps.max() <= 48 and forced > 0.02 * len(steps)
+    f, _, nlab = CV.train_field(bvh, v, bc, wp, tp, {"n_levels": 3, "base_resolution": 8, "growth": 2.0},
+                                label_stride=2, path_len=32, rounds=1, train_steps=(40, 60), res=64, design=6,
+                                seed=11, solver=cs, label_max_steps=48)
+    assert len(nlab) == 2 and min(nlab) > 0
+    st = cs.solve(tp, 16, f, seed=77)
+    pl = W.Solver(bvh, bc, wp).solve(tp, 16, seed=77)
+    torch.cuda.synchronize()
+    assert int(st.forced.sum().item()) == int(pl.forced.sum().item()) == 0
+    mean, var = st.mean.cpu().numpy(), st.variance().cpu().numpy()
+    vp = float(pl.variance().mean().item())
+    sd = np.sqrt(var / 16)
+    ok = sd > 0
+    z = (mean[ok] - truth[ok]) / sd[ok]
+    me = float(np.mean(mean - truth))
+    se = float(np.sqrt(np.sum(var / 16)) / len(var))
+    print("cv capped labels: forced %d of %d, var %.3g (plain %.3g, %.1fx), mean error %.3g = %.2f se, z sd %.3f"
+          % (forced, len(steps), var.mean(), vp, vp / var.mean(), me, me / se, z.std()))
+    assert ok.mean() > 0.99
+    assert abs(me) < 5 * se
+    assert np.mean(np.abs(z) > 4) < 0.01 and 0.85 < z.std() < 1.15
+    assert var.mean() < vp / 5
