@@ -89,10 +89,11 @@ def parse():
                     help="run a 1 GiB device copy (counter calibration) and one walker solve, then exit; "
                          "used under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--source-hash", action="store_true", help="print the walker sources' hash and exit")
-    ap.add_argument("--section", choices=["walker", "cache", "closest_stress", "poisson"], default=None,
+    ap.add_argument("--section", choices=["walker", "cache", "closest_stress", "poisson", "cv"], default=None,
                     help="run one section's GPU work only (one headline solve; the cache's training steps and "
-                         "inference; the closest-point stress sets; problem 3), print a short JSON line and "
-                         "exit: the workload of the counter profiles")
+                         "inference; the closest-point stress sets; problem 3; the control variate's training "
+                         "pipeline and its 2-walk estimate, three times), print a short JSON line and exit: the "
+                         "workload of the profiles")
     return ap.parse_args()
 
 
@@ -358,6 +359,32 @@ def cache_rates(eng, torch) -> dict:
             "train_mfma_frac_fp16_dense": tflops_train / 2500.0, "infer_mfma_frac_fp16_dense": tflops_inf / 2500.0}
 
 
+def cv_pipeline(torch, bvh, verts, tp, bc, params, pts, reps: int = 3):
+    """The control variate's first time-to-RMSE setup on problem 2: the
+    training pipeline (capped label solves, Adam steps, bakes) and the
+    estimate at its first walk count, `reps` times; host-clock seconds of
+    each part of the last repetition and the estimate's RMSE against FP64."""
+    from n2wos_amd import cv as CV, problems as P, ttrmse
+    su = ttrmse.DEFAULT_CV_SETUPS[0]
+    cs = CV.CvSolver(bvh, bc, params)
+    truth = P.charges_solution_f64(bc.charges, pts)
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fld, cache, _ = CV.train_field(bvh, verts, bc, params, tp, su.cache_kw, su.label_stride, su.path_len,
+                                       su.rounds, su.train_steps, su.batch, su.res, su.points, 11, solver=cs,
+                                       label_max_steps=su.label_max_steps)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        st = cs.solve(tp, su.wpp[0], fld, seed=1000 + su.wpp[0])
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        del cache
+    rmse = float(np.sqrt(np.mean((st.mean.cpu().numpy() - truth) ** 2)))
+    return {"setup": su.name, "train_seconds": t1 - t0, "estimate_seconds": t2 - t1, "walks_per_point": su.wpp[0],
+            "rmse": rmse}
+
+
 def main():
     args = parse()
     if args.source_hash:
@@ -399,6 +426,9 @@ def main():
         return
     if args.section == "poisson":
         print(json.dumps({"section": "poisson", **poisson_problem3(eng, torch, P, W)}))
+        return
+    if args.section == "cv":
+        print(json.dumps({"section": "cv", **cv_pipeline(torch, bvh, verts, tp, bc, params, pts)}))
         return
 
     if args.traffic_probe:
